@@ -1,0 +1,237 @@
+"""Python front end of libcfhip.so for tests, the benchmark and torch autograd.
+
+Device buffers are torch tensors on ``cuda:0`` (torch supplies device memory and
+the stream); every computation runs in the HIP kernels of libcfhip.so. Importing
+this package fails if the library has not been built -- there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import (CF_PREC_F32C, CF_PREC_F64, CF_SPEC_DERIV_POWER_CUTOFF, CF_SPEC_GAUSS,  # noqa: F401
+                   CF_SPEC_KM3_GAUSS, CF_SPEC_POWER_CUTOFF, CF_SPEC_SCALE_INV, CfError, Cosmology,
+                   KGParams, check)
+
+lib = _lib.load()
+LIB_PATH = _lib.LIB_PATH
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if isinstance(t, torch.Tensor):
+        assert t.is_cuda and t.is_contiguous(), "device tensors must be contiguous CUDA tensors"
+        return C.c_void_p(t.data_ptr())
+    return C.c_void_p(int(t))
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def minkowski():
+    """StaticEOSCosmology() (src/physics.hpp:30-31)."""
+    return Cosmology(1.0, 0.0, 0.0, 1.0)
+
+
+def radiation(a1, H1, t1):
+    """StaticEOSCosmology(param) with p = 1 (src/physics.hpp:25-27)."""
+    return Cosmology(a1, H1, t1, 1.0)
+
+
+def kg_params(N, L, m, lam=0.0, cosmo=None):
+    return KGParams(int(N), float(L), float(m), float(lam), cosmo if cosmo is not None else minkowski())
+
+
+def cosmo_a(c, t):
+    return lib.cf_cosmo_a(C.byref(c), t)
+
+
+def cosmo_H(c, t):
+    return lib.cf_cosmo_H(C.byref(c), t)
+
+
+def cosmo_eta(c, t):
+    return lib.cf_cosmo_eta(C.byref(c), t)
+
+
+def inv_ah_sqr(N, L, a_t):
+    return lib.cf_inv_ah_sqr(N, L, a_t)
+
+
+def nbins(N):
+    return int(lib.cf_spectrum_bins(N))
+
+
+def sync(stream=None):
+    check(lib.cf_sync(_stream(stream)))
+
+
+# ---------------------------------------------------------------- odeint system / ops
+def rhs_f64(N, x, dxdt, m, lam, H, inv, stream=None):
+    check(lib.cf_rhs_f64(N, _ptr(x), _ptr(dxdt), m, lam, H, inv, _stream(stream)))
+
+
+def scale_sum(v0, vs, alphas, stream=None):
+    K = len(vs)
+    arr = (C.c_void_p * K)(*[t.data_ptr() for t in vs])
+    al = (C.c_double * K)(*alphas)
+    check(lib.cf_scale_sum(K, _ptr(v0), arr, al, v0.numel(), _stream(stream)))
+
+
+def norm_inf(v, stream=None):
+    out = C.c_double()
+    check(lib.cf_norm_inf(_ptr(v), v.numel(), C.byref(out), _stream(stream)))
+    return out.value
+
+
+def energy_density_f64(N, x, rho, m, lam, inv, stream=None):
+    check(lib.cf_energy_density_f64(N, _ptr(x), _ptr(rho), m, lam, inv, _stream(stream)))
+
+
+# ---------------------------------------------------------------- fused RK4
+def work_bytes(N, precision=CF_PREC_F64):
+    return int(lib.cf_rk4_work_bytes(N, precision))
+
+
+def alloc_work(N, precision=CF_PREC_F64, device="cuda"):
+    return torch.empty(work_bytes(N, precision), dtype=torch.uint8, device=device)
+
+
+def rk4_step_f64(p, x, t, dt, work, stream=None):
+    check(lib.cf_rk4_step_f64(C.byref(p), _ptr(x), t, dt, _ptr(work), _stream(stream)))
+
+
+def rk4_run_f64(p, x, t0, k0, n_steps, dt, work, stream=None):
+    check(lib.cf_rk4_run_f64(C.byref(p), _ptr(x), t0, k0, n_steps, dt, _ptr(work), _stream(stream)))
+
+
+def rk4_step_f32c(p, hi, lo, t, dt, work, stream=None):
+    check(lib.cf_rk4_step_f32c(C.byref(p), _ptr(hi), _ptr(lo), t, dt, _ptr(work), _stream(stream)))
+
+
+def rk4_run_f32c(p, hi, lo, t0, k0, n_steps, dt, work, stream=None):
+    check(lib.cf_rk4_run_f32c(C.byref(p), _ptr(hi), _ptr(lo), t0, k0, n_steps, dt, _ptr(work),
+                              _stream(stream)))
+
+
+def split_f64(x, stream=None):
+    hi = torch.empty(x.numel(), dtype=torch.float32, device=x.device)
+    lo = torch.empty_like(hi)
+    check(lib.cf_split_f64(_ptr(x), _ptr(hi), _ptr(lo), x.numel(), _stream(stream)))
+    return hi, lo
+
+
+def join_f64(hi, lo, stream=None):
+    x = torch.empty(hi.numel(), dtype=torch.float64, device=hi.device)
+    check(lib.cf_join_f64(_ptr(hi), _ptr(lo), _ptr(x), hi.numel(), _stream(stream)))
+    return x
+
+
+# ---------------------------------------------------------------- spectra
+class SpectrumPlan:
+    """hipFFT plans + bin table for an N^3 lattice (cufftWrapperNoBatching analogue)."""
+
+    def __init__(self, N):
+        self.N = int(N)
+        h = C.c_void_p()
+        check(lib.cf_spectrum_plan_create(self.N, C.byref(h)))
+        self.h = h
+        self.nb = nbins(self.N)
+
+    def close(self):
+        if self.h:
+            lib.cf_spectrum_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _bins(self, out):
+        if out is None:
+            out = torch.empty(self.nb, dtype=torch.float64, device="cuda")
+        return out
+
+    def d2z(self, f, out=None, stream=None):
+        N = self.N
+        if out is None:
+            out = torch.empty(2 * N * N * (N // 2 + 1), dtype=torch.float64, device="cuda")
+        check(lib.cf_fft_d2z(self.h, _ptr(f), _ptr(out), _stream(stream)))
+        return out
+
+    def z2d(self, fk, out=None, stream=None):
+        """Unnormalised C2R; fk is destroyed."""
+        if out is None:
+            out = torch.empty(self.N ** 3, dtype=torch.float64, device="cuda")
+        check(lib.cf_fft_z2d(self.h, _ptr(fk), _ptr(out), _stream(stream)))
+        return out
+
+    def power_spectrum(self, f, out=None, stream=None):
+        out = self._bins(out)
+        check(lib.cf_power_spectrum(self.h, _ptr(f), _ptr(out), _stream(stream)))
+        return out
+
+    def mode_power_spectrum(self, state, L, m, a_t, out=None, stream=None):
+        out = self._bins(out)
+        check(lib.cf_mode_power_spectrum(self.h, _ptr(state), L, m, a_t, _ptr(out), _stream(stream)))
+        return out
+
+    def bin_power(self, fk, out=None, stream=None):
+        out = self._bins(out)
+        check(lib.cf_bin_power(self.h, _ptr(fk), _ptr(out), _stream(stream)))
+        return out
+
+    def bin_mode_power(self, phik, dphik, L, m, a_t, out=None, stream=None):
+        out = self._bins(out)
+        check(lib.cf_bin_mode_power(self.h, _ptr(phik), _ptr(dphik), L, m, a_t, _ptr(out),
+                                    _stream(stream)))
+        return out
+
+    def radial_bins(self, sbins, out=None, stream=None):
+        if out is None:
+            out = torch.empty(self.N // 2, dtype=torch.float64, device="cuda")
+        check(lib.cf_radial_bins(self.h, _ptr(sbins), _ptr(out), _stream(stream)))
+        return out
+
+    def grf(self, w, sqrtP, Psi=None, out=None, stream=None):
+        if out is None:
+            out = torch.empty(self.N ** 3, dtype=torch.float64, device="cuda")
+        check(lib.cf_grf(self.h, _ptr(w), _ptr(sqrtP), _ptr(Psi), _ptr(out), _stream(stream)))
+        return out
+
+    def init_grf_state(self, seed, kind, params, L, rms, state=None, stream=None):
+        if state is None:
+            state = torch.empty(2 * self.N ** 3, dtype=torch.float64, device="cuda")
+        prm = np.zeros(8)
+        prm[: len(params)] = params
+        check(lib.cf_init_grf_state(self.h, int(seed), int(kind), prm.ctypes.data_as(C.c_void_p), L, rms,
+                                    _ptr(state), _stream(stream)))
+        return state
+
+
+def normals_host(seed, n):
+    out = np.empty(n)
+    check(lib.cf_normals_host(int(seed), int(n), out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
+def spectrum_table_host(kind, N, L, params=()):
+    out = np.empty(nbins(N))
+    prm = np.zeros(8)
+    prm[: len(params)] = params
+    check(lib.cf_spectrum_table_host(int(kind), int(N), float(L), prm.ctypes.data_as(C.c_void_p),
+                                     out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
+def rescale_rms(x, target, stream=None):
+    check(lib.cf_rescale_rms(_ptr(x), x.numel(), target, _stream(stream)))
+    return x

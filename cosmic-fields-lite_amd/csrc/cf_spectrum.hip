@@ -1,0 +1,492 @@
+// cf_spectrum.hip -- hipFFT wrapper, deterministic spectrum binning, Gaussian
+// random fields and the fixed-order rms rescale.
+//
+// Binning without atomics. The half spectrum has N*N*(N/2+1) entries; entry
+// (a,b,c) belongs to bin s = a'^2 + b'^2 + c^2 with x' = min(x, N-x), and counts
+// twice unless c is 0 or N/2 (the full-cube loop of fdm3d.cpp:11-28 visits the
+// mirrored column too). At plan creation the host sorts the entry indices by s
+// (a stable counting sort, so each bin lists its entries in memory order). The
+// binning kernels give one wavefront to each bin: lane j sums entries j, j+64,
+// ... of the bin, then the wavefront reduces with a fixed butterfly. The result
+// does not depend on scheduling, so two runs give identical bits.
+#include <hipfft/hipfft.h>
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "cf_common.hpp"
+
+struct cf_spectrum_plan {
+  int64_t N = 0, H = 0, nb = 0, nhalf = 0;
+  hipfftHandle d2z = 0, z2d = 0;
+  bool have_d2z = false, have_z2d = false;
+  void *work = nullptr;
+  double *rbuf = nullptr;        // N^3 real scratch
+  double *cbuf0 = nullptr;       // half-spectrum scratch (2*nhalf doubles)
+  double *cbuf1 = nullptr;       // second half-spectrum scratch
+  int64_t *bin_off = nullptr;    // nb+1 offsets into bin_idx
+  uint32_t *bin_idx = nullptr;   // nhalf entry indices sorted by bin
+  double *red = nullptr;         // reduction scratch (RED_BLOCKS + 1 doubles)
+};
+
+namespace cf {
+
+constexpr int RED_BLOCKS = 1024;
+
+#define CF_FFT(expr)                                                        \
+  do {                                                                      \
+    hipfftResult r_ = (expr);                                               \
+    if (r_ != HIPFFT_SUCCESS) {                                             \
+      ::cf::set_error("%s failed: hipfft status %d (%s:%d)", #expr, (int)r_, \
+                      __FILE__, __LINE__);                                  \
+      return CF_E_FFT;                                                      \
+    }                                                                       \
+  } while (0)
+
+__device__ __forceinline__ double wave_sum(double v) {
+  // fixed butterfly over the 64 lanes of a wavefront
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_bin_power(const double2 *__restrict__ fk,
+                                                   const int64_t *__restrict__ off,
+                                                   const uint32_t *__restrict__ idx, int64_t H,
+                                                   int64_t nb, double *__restrict__ bins) {
+  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (s >= nb) return;
+  const int64_t beg = off[s], end = off[s + 1];
+  double acc = 0.0;
+  for (int64_t j = beg + lane; j < end; j += 64) {
+    const uint32_t e = idx[j];
+    const int64_t c = (int64_t)e % H;
+    const double2 v = fk[e];
+    const double p = v.x * v.x + v.y * v.y;
+    acc += (c == 0 || c == H - 1) ? p : 2.0 * p;
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) bins[s] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_bin_mode_power(const double2 *__restrict__ pk,
+                                                        const double2 *__restrict__ dk,
+                                                        const int64_t *__restrict__ off,
+                                                        const uint32_t *__restrict__ idx, int64_t H,
+                                                        int64_t nb, double m2, double kir2, double a2,
+                                                        double *__restrict__ bins) {
+  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (s >= nb) return;
+  // fdm3d.cpp:64: m*m + (2*pi/L)*(2*pi/L)*s/(a_t*a_t)
+  const double denom = m2 + (kir2 * (double)s) / a2;
+  const int64_t beg = off[s], end = off[s + 1];
+  double acc = 0.0;
+  for (int64_t j = beg + lane; j < end; j += 64) {
+    const uint32_t e = idx[j];
+    const int64_t c = (int64_t)e % H;
+    const double2 u = pk[e], w = dk[e];
+    const double p = (u.x * u.x + u.y * u.y) + (w.x * w.x + w.y * w.y) / denom;
+    acc += (c == 0 || c == H - 1) ? p : 2.0 * p;
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) bins[s] = acc;
+}
+
+// radial[j] = sum_{s : isqrt(s) = j} sbins[s], j < N/2, summed in s order.
+__global__ void k_radial(const double *__restrict__ sb, int64_t nb, int64_t nr, double *__restrict__ rad) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nr) return;
+  int64_t lo = j * j, hi = (j + 1) * (j + 1);
+  if (hi > nb) hi = nb;
+  double acc = 0.0;
+  for (int64_t s = lo; s < hi; ++s) acc += sb[s];
+  rad[j] = acc;
+}
+
+// Multiply the half spectrum by sqrt(P) per bin (random_field.cpp:60-73).
+__global__ void k_grf_scale(double2 *__restrict__ fk, const double *__restrict__ sqrtP, int64_t N,
+                            int64_t H) {
+  const int64_t total = N * N * H;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = e % H;
+    const int64_t r = e / H;
+    const int64_t b = r % N, a = r / N;
+    const int64_t as = (a <= N / 2) ? a : a - N;
+    const int64_t bs = (b <= N / 2) ? b : b - N;
+    const int64_t s = as * as + bs * bs + c * c;
+    const double sp = sqrtP[s];
+    double2 v = fk[e];
+    v.x *= sp;
+    v.y *= sp;
+    fk[e] = v;
+  }
+}
+
+__global__ void k_mul_exp(double *__restrict__ w, const double *__restrict__ psi, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    w[i] = w[i] * exp(psi[i]);
+}
+
+// Fixed-order sum of squares: RED_BLOCKS blocks of 256 threads, grid-stride
+// partials, block tree, then one block over the partials. The grid is fixed,
+// so the order of additions is the same on every run and every device.
+__global__ __launch_bounds__(256) void k_sumsq_partial(const double *__restrict__ x, int64_t n,
+                                                       double *__restrict__ part) {
+  __shared__ double sm[256];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * 256)
+    acc += x[i] * x[i];
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+__global__ __launch_bounds__(256) void k_rms_factor(double *__restrict__ part, int64_t n, double target) {
+  __shared__ double sm[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < RED_BLOCKS; i += 256) acc += part[i];
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[RED_BLOCKS] = target / sqrt(sm[0] / (double)n);
+}
+
+__global__ void k_scale_by(double *__restrict__ x, int64_t n, const double *__restrict__ f) {
+  const double s = *f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = x[i] * s;
+}
+
+static unsigned grid_cap(int64_t n, int block) {
+  int64_t g = cdiv(n, block);
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+static cf_status plan_check(const cf_spectrum_plan *p) {
+  CF_REQUIRE(p && p->N > 0, "null or empty spectrum plan");
+  return CF_OK;
+}
+
+static cf_status d2z_exec(cf_spectrum_plan *p, double *in, double *out, hipStream_t st) {
+  CF_FFT(hipfftSetStream(p->d2z, st));
+  CF_FFT(hipfftExecD2Z(p->d2z, (hipfftDoubleReal *)in, (hipfftDoubleComplex *)out));
+  return CF_OK;
+}
+
+static cf_status z2d_exec(cf_spectrum_plan *p, double *in, double *out, hipStream_t st) {
+  CF_FFT(hipfftSetStream(p->z2d, st));
+  CF_FFT(hipfftExecZ2D(p->z2d, (hipfftDoubleComplex *)in, (hipfftDoubleReal *)out));
+  return CF_OK;
+}
+
+// Spectrum factories over s, |k| = sqrt(s)*2*pi/L (random_field.cpp:67,212-254).
+static void spectrum_table(int kind, int64_t N, double L, const double *prm, double *P) {
+  const double pi = 3.14159265358979323846;
+  const int64_t nb = 3 * (N / 2) * (N / 2) + 1;
+  double P_k_ast = 0.0, k_ast = 0.0, alpha = 0.0, P_k0 = 0.0;
+  if (kind == CF_SPEC_POWER_CUTOFF || kind == CF_SPEC_DERIV_POWER_CUTOFF) {
+    // power_law_with_cutoff_given_amplitude_3d, random_field.cpp:212-221
+    const double sigma = prm[0];
+    k_ast = prm[1];
+    alpha = prm[2];
+    P_k_ast = alpha == -3.0
+                  ? 2 * pi * pi * sigma * sigma / (k_ast * k_ast * k_ast * log(k_ast * L / (2 * pi)))
+                  : (3 + alpha) * 2 * pi * pi * sigma * sigma * (L * L * L * pow(k_ast * L, alpha)) /
+                        (pow(k_ast * L, alpha + 3) - pow(2 * pi, alpha + 3));
+    P_k_ast *= (N / L) * (N / L) * (N / L);
+  }
+  if (kind == CF_SPEC_SCALE_INV) {
+    // scale_invariant_spectrum_3d, random_field.cpp:234-240
+    const double k0 = 1.0;
+    P_k0 = prm[0] / ((k0 * k0 * k0) / (2 * pi * pi));
+    P_k0 *= (N / L) * (N / L) * (N / L);
+  }
+  for (int64_t s = 0; s < nb; ++s) {
+    const double k = sqrt((double)s) * 2 * pi / L;
+    double v = 0.0;
+    switch (kind) {
+      case CF_SPEC_GAUSS: v = (k == 0.0) ? 0.0 : exp(-(k * k) / (2 * prm[0] * prm[0])); break;
+      case CF_SPEC_KM3_GAUSS: v = (k == 0.0) ? 0.0 : pow(k, -3) * exp(-(k * k) / 2); break;
+      case CF_SPEC_SCALE_INV: v = (k == 0.0) ? 0.0 : (P_k0 * pow(k / 1.0, -3)); break;
+      case CF_SPEC_POWER_CUTOFF:
+        v = (k == 0.0) ? 0.0 : (P_k_ast * (k <= k_ast ? pow(k / k_ast, alpha) : 0.0));
+        break;
+      case CF_SPEC_DERIV_POWER_CUTOFF: {
+        // to_deriv_spectrum, random_field.cpp:248-250
+        const double m = prm[3];
+        const double Pf = (k == 0.0) ? 0.0 : (P_k_ast * (k <= k_ast ? pow(k / k_ast, alpha) : 0.0));
+        v = (k * k + m * m) * Pf;
+        break;
+      }
+      default: v = 0.0;
+    }
+    P[s] = v;
+  }
+}
+
+}  // namespace cf
+
+using namespace cf;
+
+extern "C" {
+
+int64_t cf_spectrum_bins(int64_t N) { return 3 * (N / 2) * (N / 2) + 1; }
+
+cf_status cf_spectrum_plan_destroy(cf_spectrum_plan *p) {
+  if (!p) return CF_OK;
+  if (p->have_d2z) hipfftDestroy(p->d2z);
+  if (p->have_z2d) hipfftDestroy(p->z2d);
+  void *bufs[] = {p->work, p->rbuf, p->cbuf0, p->cbuf1, p->bin_off, p->bin_idx, p->red};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  delete p;
+  return CF_OK;
+}
+
+cf_status cf_spectrum_plan_create(int64_t N, cf_spectrum_plan **out) {
+  CF_REQUIRE(out, "cf_spectrum_plan_create: null output");
+  *out = nullptr;
+  CF_REQUIRE(valid_N(N) && N <= 1024, "cf_spectrum_plan_create: unsupported N=%lld", (long long)N);
+  cf_spectrum_plan *p = new cf_spectrum_plan();
+  p->N = N;
+  p->H = N / 2 + 1;
+  p->nb = cf_spectrum_bins(N);
+  p->nhalf = N * N * p->H;
+  auto fail = [&](cf_status s) {
+    cf_spectrum_plan_destroy(p);
+    return s;
+  };
+  // hipFFT plans sharing one work area (cuda_wrapper.cu:202-222)
+  size_t ws1 = 0, ws2 = 0;
+  if (hipfftCreate(&p->d2z) != HIPFFT_SUCCESS) { set_error("hipfftCreate failed"); return fail(CF_E_FFT); }
+  p->have_d2z = true;
+  if (hipfftCreate(&p->z2d) != HIPFFT_SUCCESS) { set_error("hipfftCreate failed"); return fail(CF_E_FFT); }
+  p->have_z2d = true;
+  hipfftSetAutoAllocation(p->d2z, 0);
+  hipfftSetAutoAllocation(p->z2d, 0);
+  if (hipfftMakePlan3d(p->d2z, (int)N, (int)N, (int)N, HIPFFT_D2Z, &ws1) != HIPFFT_SUCCESS ||
+      hipfftMakePlan3d(p->z2d, (int)N, (int)N, (int)N, HIPFFT_Z2D, &ws2) != HIPFFT_SUCCESS) {
+    set_error("hipfftMakePlan3d(N=%lld) failed", (long long)N);
+    return fail(CF_E_FFT);
+  }
+  const size_t ws = ws1 > ws2 ? ws1 : ws2;
+  const size_t n3 = (size_t)N * N * N;
+  if ((ws && hipMalloc(&p->work, ws) != hipSuccess) ||
+      hipMalloc((void **)&p->rbuf, n3 * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&p->cbuf0, 2 * (size_t)p->nhalf * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&p->cbuf1, 2 * (size_t)p->nhalf * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&p->bin_off, ((size_t)p->nb + 1) * sizeof(int64_t)) != hipSuccess ||
+      hipMalloc((void **)&p->bin_idx, (size_t)p->nhalf * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void **)&p->red, (RED_BLOCKS + 1) * sizeof(double)) != hipSuccess) {
+    set_error("cf_spectrum_plan_create(N=%lld): device allocation failed", (long long)N);
+    return fail(CF_E_NOMEM);
+  }
+  if (ws) {
+    if (hipfftSetWorkArea(p->d2z, p->work) != HIPFFT_SUCCESS ||
+        hipfftSetWorkArea(p->z2d, p->work) != HIPFFT_SUCCESS) {
+      set_error("hipfftSetWorkArea failed");
+      return fail(CF_E_FFT);
+    }
+  }
+  // Bin table: stable counting sort of half-spectrum entries by s.
+  {
+    const int64_t H = p->H, nb = p->nb;
+    std::vector<int64_t> off((size_t)nb + 1, 0);
+    std::vector<uint32_t> sq((size_t)N);
+    for (int64_t a = 0; a < N; ++a) {
+      const int64_t as = (a <= N / 2) ? a : N - a;
+      sq[a] = (uint32_t)(as * as);
+    }
+    for (int64_t a = 0; a < N; ++a)
+      for (int64_t b = 0; b < N; ++b) {
+        const int64_t r = sq[a] + sq[b];
+        for (int64_t c = 0; c < H; ++c) off[r + c * c + 1]++;
+      }
+    for (int64_t s = 0; s < nb; ++s) off[s + 1] += off[s];
+    std::vector<int64_t> pos(off.begin(), off.end() - 1);
+    std::vector<uint32_t> idx((size_t)p->nhalf);
+    uint32_t e = 0;
+    for (int64_t a = 0; a < N; ++a)
+      for (int64_t b = 0; b < N; ++b) {
+        const int64_t r = sq[a] + sq[b];
+        for (int64_t c = 0; c < H; ++c, ++e) idx[pos[r + c * c]++] = e;
+      }
+    if (hipMemcpy(p->bin_off, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->bin_idx, idx.data(), idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("cf_spectrum_plan_create: bin table upload failed");
+      return fail(CF_E_HIP);
+    }
+  }
+  *out = p;
+  return CF_OK;
+}
+
+cf_status cf_fft_d2z(cf_spectrum_plan *p, const double *in, double *out, cf_stream stream) {
+  CF_TRY(plan_check(p));
+  CF_REQUIRE(in && out, "cf_fft_d2z: null buffer");
+  const size_t n3 = (size_t)p->N * p->N * p->N;
+  CF_HIP(hipMemcpyAsync(p->rbuf, in, n3 * sizeof(double), hipMemcpyDeviceToDevice, S(stream)));
+  return d2z_exec(p, p->rbuf, out, S(stream));
+}
+
+cf_status cf_fft_z2d(cf_spectrum_plan *p, double *in, double *out, cf_stream stream) {
+  CF_TRY(plan_check(p));
+  CF_REQUIRE(in && out, "cf_fft_z2d: null buffer");
+  return z2d_exec(p, in, out, S(stream));
+}
+
+cf_status cf_bin_power(cf_spectrum_plan *p, const double *fk, double *bins, cf_stream stream) {
+  CF_TRY(plan_check(p));
+  CF_REQUIRE(fk && bins, "cf_bin_power: null buffer");
+  hipLaunchKernelGGL(k_bin_power, dim3((unsigned)cdiv(p->nb, 4)), dim3(256), 0, S(stream),
+                     (const double2 *)fk, p->bin_off, p->bin_idx, p->H, p->nb, bins);
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
+cf_status cf_bin_mode_power(cf_spectrum_plan *p, const double *phik, const double *dphik, double L,
+                            double m, double a_t, double *bins, cf_stream stream) {
+  CF_TRY(plan_check(p));
+  CF_REQUIRE(phik && dphik && bins, "cf_bin_mode_power: null buffer");
+  const double pi = 3.14159265358979323846;
+  const double kir2 = (2 * pi / L) * (2 * pi / L);
+  hipLaunchKernelGGL(k_bin_mode_power, dim3((unsigned)cdiv(p->nb, 4)), dim3(256), 0, S(stream),
+                     (const double2 *)phik, (const double2 *)dphik, p->bin_off, p->bin_idx, p->H, p->nb,
+                     m * m, kir2, a_t * a_t, bins);
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
+cf_status cf_power_spectrum(cf_spectrum_plan *p, const double *f, double *bins, cf_stream stream) {
+  CF_TRY(plan_check(p));
+  CF_TRY(cf_fft_d2z(p, f, p->cbuf0, stream));
+  return cf_bin_power(p, p->cbuf0, bins, stream);
+}
+
+cf_status cf_mode_power_spectrum(cf_spectrum_plan *p, const double *state, double L, double m,
+                                 double a_t, double *bins, cf_stream stream) {
+  CF_TRY(plan_check(p));
+  CF_REQUIRE(state && bins, "cf_mode_power_spectrum: null buffer");
+  const int64_t N3 = p->N * p->N * p->N;
+  CF_TRY(cf_fft_d2z(p, state, p->cbuf0, stream));
+  CF_TRY(cf_fft_d2z(p, state + N3, p->cbuf1, stream));
+  return cf_bin_mode_power(p, p->cbuf0, p->cbuf1, L, m, a_t, bins, stream);
+}
+
+cf_status cf_radial_bins(cf_spectrum_plan *p, const double *sbins, double *radial, cf_stream stream) {
+  CF_TRY(plan_check(p));
+  CF_REQUIRE(sbins && radial, "cf_radial_bins: null buffer");
+  const int64_t nr = p->N / 2;
+  hipLaunchKernelGGL(k_radial, dim3((unsigned)cdiv(nr, 64)), dim3(64), 0, S(stream), sbins, p->nb, nr,
+                     radial);
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
+cf_status cf_normals_host(uint32_t seed, int64_t n, double *out) {
+  CF_REQUIRE(out && n >= 0, "cf_normals_host: bad arguments");
+  std::mt19937 mt{seed};
+  std::normal_distribution<double> dist{0.0, 1.0};
+  for (int64_t i = 0; i < n; ++i) out[i] = dist(mt);
+  return CF_OK;
+}
+
+cf_status cf_spectrum_table_host(int kind, int64_t N, double L, const double *params, double *P) {
+  CF_REQUIRE(P && valid_N(N) && L > 0, "cf_spectrum_table_host: bad arguments");
+  CF_REQUIRE(kind >= 0 && kind <= 4, "cf_spectrum_table_host: unknown spectrum kind %d", kind);
+  CF_REQUIRE(kind == CF_SPEC_KM3_GAUSS || params, "cf_spectrum_table_host: missing parameters");
+  double zero[4] = {0, 0, 0, 0};
+  spectrum_table(kind, N, L, params ? params : zero, P);
+  return CF_OK;
+}
+
+cf_status cf_grf(cf_spectrum_plan *p, double *w, const double *sqrtP, const double *Psi, double *out,
+                 cf_stream stream) {
+  CF_TRY(plan_check(p));
+  CF_REQUIRE(w && sqrtP && out, "cf_grf: null buffer");
+  const int64_t N3 = p->N * p->N * p->N;
+  hipStream_t st = S(stream);
+  if (Psi) {
+    hipLaunchKernelGGL(k_mul_exp, dim3(grid_cap(N3, 256)), dim3(256), 0, st, w, Psi, N3);
+    CF_LAUNCHED();
+  }
+  CF_TRY(d2z_exec(p, w, p->cbuf0, st));
+  hipLaunchKernelGGL(k_grf_scale, dim3(grid_cap(p->nhalf, 256)), dim3(256), 0, st, (double2 *)p->cbuf0,
+                     sqrtP, p->N, p->H);
+  CF_LAUNCHED();
+  return z2d_exec(p, p->cbuf0, out, st);
+}
+
+cf_status cf_rescale_rms(double *x, int64_t n, double target, cf_stream stream) {
+  CF_REQUIRE(x && n > 0, "cf_rescale_rms: bad arguments");
+  // Uses a small private scratch; allocate per call (not a hot path).
+  double *red = nullptr;
+  CF_HIP(hipMallocAsync((void **)&red, (RED_BLOCKS + 1) * sizeof(double), S(stream)));
+  hipLaunchKernelGGL(k_sumsq_partial, dim3(RED_BLOCKS), dim3(256), 0, S(stream), x, n, red);
+  hipLaunchKernelGGL(k_rms_factor, dim3(1), dim3(256), 0, S(stream), red, n, target);
+  hipLaunchKernelGGL(k_scale_by, dim3(grid_cap(n, 256)), dim3(256), 0, S(stream), x, n, red + RED_BLOCKS);
+  hipError_t le = hipGetLastError();
+  (void)hipFreeAsync(red, S(stream));
+  CF_HIP(le);
+  return CF_OK;
+}
+
+cf_status cf_init_grf_state(cf_spectrum_plan *p, uint32_t seed, int kind, const double *params, double L,
+                            double rms, double *state, cf_stream stream) {
+  CF_TRY(plan_check(p));
+  CF_REQUIRE(state, "cf_init_grf_state: null state");
+  const int64_t N = p->N, N3 = N * N * N;
+  hipStream_t st = S(stream);
+  std::vector<double> P((size_t)p->nb);
+  CF_TRY(cf_spectrum_table_host(kind, N, L, params, P.data()));
+  for (auto &v : P) v = sqrt(v);
+  double *host = nullptr;
+  CF_HIP(hipHostMalloc((void **)&host, (size_t)N3 * sizeof(double), hipHostMallocDefault));
+  {
+    // random_field.cpp:44-46: w_i = (1/N^3) * normal_i, in index order
+    std::mt19937 mt{seed};
+    std::normal_distribution<double> dist{0.0, 1.0};
+    const double scale = 1.0 / (double)(N * N * N);
+    for (int64_t i = 0; i < N3; ++i) host[i] = scale * dist(mt);
+  }
+  double *sqrtP = nullptr;
+  cf_status rc = CF_OK;
+  if (hipMalloc((void **)&sqrtP, P.size() * sizeof(double)) != hipSuccess) {
+    (void)hipHostFree(host);
+    set_error("cf_init_grf_state: allocation failed");
+    return CF_E_NOMEM;
+  }
+  // the white noise goes to the pi half of the state, which is zeroed afterwards
+  double *w = state + N3;
+  if (hipMemcpyAsync(w, host, (size_t)N3 * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(sqrtP, P.data(), P.size() * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) {
+    rc = CF_E_HIP;
+    set_error("cf_init_grf_state: upload failed");
+  }
+  if (rc == CF_OK) rc = cf_grf(p, w, sqrtP, nullptr, state, stream);
+  if (rc == CF_OK && rms > 0.0) rc = cf_rescale_rms(state, N3, rms, stream);
+  if (rc == CF_OK) rc = cf_fill_zero(w, (size_t)N3 * sizeof(double), stream);
+  hipError_t se = hipStreamSynchronize(st);
+  (void)hipFree(sqrtP);
+  (void)hipHostFree(host);
+  if (rc != CF_OK) return rc;
+  CF_HIP(se);
+  return CF_OK;
+}
+
+}  // extern "C"

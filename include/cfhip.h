@@ -1,0 +1,196 @@
+/* cfhip.h -- C ABI of libcfhip.so, the MI355X (gfx950) lattice-evolution library.
+ *
+ * Drop-in boundary for the Cosmic-Fields-Lite hot path: every entry point below
+ * names the reference interface it replaces (paths relative to the reference
+ * repository). Plain pointers and sizes only; no C++ or torch types cross.
+ *
+ * Conventions
+ *   - Lattice site (a,b,c) of an N^3 lattice is at N*N*a + N*b + c (c fastest),
+ *     as IDX_OF in src/fdm3d.hpp:16. A state is [phi (N^3) | dphi/dt (N^3)].
+ *   - All pointers named *_dev are device pointers owned by the caller.
+ *   - Launches are asynchronous on the caller's stream (NULL = default stream);
+ *     only cf_d2h, cf_sync and the *_host helpers block.
+ *   - Every call returns CF_OK or an error code; cf_last_error() gives a
+ *     thread-local message. No exception crosses the ABI.
+ *   - Handles are not thread-safe (the reference is single-threaded).
+ */
+#ifndef CFHIP_H
+#define CFHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t cf_status;
+#define CF_OK 0
+#define CF_E_INVALID 1     /* bad argument (size, null pointer, unsupported N) */
+#define CF_E_HIP 2         /* HIP runtime error */
+#define CF_E_FFT 3         /* hipFFT error */
+#define CF_E_NOMEM 4       /* device or host allocation failed */
+#define CF_E_UNSUPPORTED 5 /* valid request this build does not implement */
+
+typedef void *cf_stream; /* a hipStream_t; NULL is the default stream */
+
+/* ---------------------------------------------------------------- status */
+const char *cf_last_error(void);
+const char *cf_version(void);
+
+/* ---------------------------------------------------------------- device, memory
+ * Replace thrust::device_vector<double> allocation/fill (src/cuda_wrapper.cuh:39-45),
+ * the initializer H2D copy ALGORITHM_NAMESPACE::copy (src/initializer.hpp:24-30,43-44)
+ * and copy_vector D2H (src/cuda_wrapper.cu:45-54). */
+cf_status cf_device_count(int *count);
+cf_status cf_set_device(int device);
+cf_status cf_sync(cf_stream stream);
+cf_status cf_stream_create(cf_stream *stream);
+cf_status cf_stream_destroy(cf_stream stream);
+cf_status cf_alloc(size_t bytes, void **ptr_dev);
+cf_status cf_free(void *ptr_dev);
+cf_status cf_h2d(void *dst_dev, const void *src_host, size_t bytes, cf_stream stream); /* completes before return */
+cf_status cf_d2h(void *dst_host, const void *src_dev, size_t bytes, cf_stream stream); /* completes before return */
+cf_status cf_d2d(void *dst_dev, const void *src_dev, size_t bytes, cf_stream stream);
+cf_status cf_fill_zero(void *ptr_dev, size_t bytes, cf_stream stream); /* a kernel, safe inside graph capture */
+
+/* ---------------------------------------------------------------- parameters */
+/* StaticEOSCosmology (src/physics.hpp:15-56): a(t), H(t), eta(t) for a constant
+ * equation of state. Minkowski = {1, 0, 0, 1} (physics.hpp:30-31). */
+typedef struct {
+  double a1, H1, t1, p;
+} cf_cosmology;
+
+/* Klein-Gordon in FRW with an optional lambda*phi^4/4 potential:
+ *   phi'' + 3H phi' - Lap(phi)/a^2 + m^2 phi + lambda phi^3 = 0
+ * KleinGordonEquation (src/equations.cpp:8-36) is cosmology = Minkowski, lambda = 0;
+ * KleinGordonEquationInFRW (equations.cpp:71-104, equations_cuda.cu:522-534) has lambda = 0;
+ * CudaLambdaEquationInFRW (equations_cuda.cu:575-607) has lambda != 0. */
+typedef struct {
+  int64_t N;
+  double L, m, lambda;
+  cf_cosmology cosmo;
+} cf_kg_params;
+
+/* Host evaluation of the cosmology (physics.hpp:34-41, 52-55) and of
+ * inv_ah_sqr = 1/((L/N)^2)/(a^2) (equations.cpp:80). */
+double cf_cosmo_a(const cf_cosmology *c, double t);
+double cf_cosmo_H(const cf_cosmology *c, double t);
+double cf_cosmo_eta(const cf_cosmology *c, double t);
+double cf_inv_ah_sqr(int64_t N, double L, double a_t);
+
+/* ---------------------------------------------------------------- odeint system and ops (FP64)
+ * cf_rhs_f64 replaces Equation::operator()(const State &x, State &dxdt, double t)
+ * (KG_FRW_equation_kernel / Lambda_FRW_equation_kernel, equations_cuda.cu:15-61, via
+ * CudaKleinGordonEquationInFRW::operator() :522-534 and CudaLambdaEquationInFRW :575-589).
+ * H and inv_ah2 are the host-evaluated H(t) and cf_inv_ah_sqr(N, L, a(t)). */
+cf_status cf_rhs_f64(int64_t N, const double *x_dev, double *dxdt_dev, double m, double lambda,
+                     double H, double inv_ah2, cf_stream stream);
+
+/* thrust_operations::scale_sumK (src/odeint_thrust/thrust_operations.cu:4-189, ops :220-353):
+ * v0 = sum_{i<K} alpha[i] * v[i], evaluated left to right, 1 <= K <= 14. */
+cf_status cf_scale_sum(int K, double *v0_dev, const double *const *v_dev_array_host,
+                       const double *alpha_host, int64_t n, cf_stream stream);
+cf_status cf_scale_sum2(double *v0_dev, const double *v1_dev, const double *v2_dev, double a1,
+                        double a2, int64_t n, cf_stream stream);
+cf_status cf_scale_sum5(double *v0_dev, const double *v1_dev, const double *v2_dev,
+                        const double *v3_dev, const double *v4_dev, const double *v5_dev, double a1,
+                        double a2, double a3, double a4, double a5, int64_t n, cf_stream stream);
+/* vector_space_norm_inf (src/odeint_thrust/thrust_algebra.hpp:51-64): max |v|, blocking. */
+cf_status cf_norm_inf(const double *v_dev, int64_t n, double *out_host, cf_stream stream);
+
+/* ---------------------------------------------------------------- fused RK4 steps
+ * One classic RK4 step (runge_kutta4_classic::do_step_impl,
+ * external/boost/numeric/odeint/stepper/runge_kutta4_classic.hpp:87-138, with the
+ * scale_sum2/scale_sum5 axpys of thrust_operations.cu:236-313) in four fused stage
+ * kernels. cf_rk4_run takes n_steps steps at t_k = t0 + (k0 + k) * dt, the time
+ * formula of integrate_const (detail/integrate_const.hpp:56-63).
+ *
+ * FP64: x_dev is the state (2N^3 doubles), updated in place; the results equal the
+ * reference arithmetic order (1*x + dt/6 k1 + dt/3 k2 + dt/3 k3 + dt/6 k4).
+ * work_dev holds cf_rk4_work_bytes(N, CF_PREC_F64) bytes. */
+#define CF_PREC_F64 0
+#define CF_PREC_F32C 1 /* float hi+lo state, float stage arrays, double arithmetic */
+size_t cf_rk4_work_bytes(int64_t N, int precision);
+cf_status cf_rk4_step_f64(const cf_kg_params *p, double *x_dev, double t, double dt, void *work_dev,
+                          cf_stream stream);
+cf_status cf_rk4_run_f64(const cf_kg_params *p, double *x_dev, double t0, int64_t k0, int64_t n_steps,
+                         double dt, void *work_dev, cf_stream stream);
+/* FP32-compensated storage: hi_dev, lo_dev are 2N^3 floats each; the state is
+ * hi + lo. Stage inputs and the RK4 increment are stored as float, all arithmetic is
+ * double, the update x += increment is compensated into lo. */
+cf_status cf_rk4_step_f32c(const cf_kg_params *p, float *hi_dev, float *lo_dev, double t, double dt,
+                           void *work_dev, cf_stream stream);
+cf_status cf_rk4_run_f32c(const cf_kg_params *p, float *hi_dev, float *lo_dev, double t0, int64_t k0,
+                          int64_t n_steps, double dt, void *work_dev, cf_stream stream);
+/* Split a double state into hi/lo floats and back (x = hi + lo). */
+cf_status cf_split_f64(const double *x_dev, float *hi_dev, float *lo_dev, int64_t n, cf_stream stream);
+cf_status cf_join_f64(const float *hi_dev, const float *lo_dev, double *x_dev, int64_t n, cf_stream stream);
+
+/* ---------------------------------------------------------------- energy density
+ * Equation::compute_energy_density (equations.cpp:106-138; energy_density_kernel,
+ * equations_cuda.cu:332-354 via :537-553 and :591-607): rho (N^3) from the state. */
+cf_status cf_energy_density_f64(int64_t N, const double *x_dev, double *rho_dev, double m,
+                                double lambda, double inv_ah2, cf_stream stream);
+
+/* ---------------------------------------------------------------- FFT and spectra
+ * A spectrum plan owns the hipFFT D2Z/Z2D plans of an N^3 lattice with one shared
+ * work area (cufftWrapperNoBatching, src/cuda_wrapper.cu:202-255), scratch buffers,
+ * and the bin table used by the deterministic binning kernels. */
+typedef struct cf_spectrum_plan cf_spectrum_plan;
+cf_status cf_spectrum_plan_create(int64_t N, cf_spectrum_plan **plan);
+cf_status cf_spectrum_plan_destroy(cf_spectrum_plan *plan);
+int64_t cf_spectrum_bins(int64_t N); /* 3*(N/2)^2 + 1 */
+
+/* execute_d2z / execute_z2d (cuda_wrapper.cu:230-235, 245-250; fftw_wrapper.cpp:24-49):
+ * out = unnormalised R2C of in (N*N*(N/2+1) interleaved complex); in is preserved.
+ * z2d: out = unnormalised C2R of in; in is destroyed (as FFTW / cuFFT). */
+cf_status cf_fft_d2z(cf_spectrum_plan *plan, const double *in_dev, double *out_dev, cf_stream stream);
+cf_status cf_fft_z2d(cf_spectrum_plan *plan, double *in_dev, double *out_dev, cf_stream stream);
+
+/* compute_power_spectrum (src/fdm3d.cpp:7-34; fdm3d_cuda.cu:161-176):
+ * bins_dev[s] = sum over the full k-cube with |n|^2 = s of |f_k|^2. Deterministic. */
+cf_status cf_power_spectrum(cf_spectrum_plan *plan, const double *f_dev, double *bins_dev,
+                            cf_stream stream);
+/* compute_mode_power_spectrum (fdm3d.cpp:36-72; fdm3d_cuda.cu:144-159):
+ * bins_dev[s] = sum |phi_k|^2 + |dphi_k|^2 / (m^2 + (2 pi/L)^2 s / a^2). */
+cf_status cf_mode_power_spectrum(cf_spectrum_plan *plan, const double *state_dev, double L, double m,
+                                 double a_t, double *bins_dev, cf_stream stream);
+/* The binning kernels alone, on half-spectra (sum_power_kernel / sum_mode_power_kernel,
+ * fdm3d_cuda.cu:3-62, without atomics). */
+cf_status cf_bin_power(cf_spectrum_plan *plan, const double *fk_dev, double *bins_dev, cf_stream stream);
+cf_status cf_bin_mode_power(cf_spectrum_plan *plan, const double *phik_dev, const double *dphik_dev,
+                            double L, double m, double a_t, double *bins_dev, cf_stream stream);
+/* Radial bins (no reference counterpart; plot_util.py:54-74 rebins s offline):
+ * radial_dev[j] = sum of s-bins with floor(sqrt(s)) = j, j < N/2. */
+cf_status cf_radial_bins(cf_spectrum_plan *plan, const double *sbins_dev, double *radial_dev,
+                         cf_stream stream);
+
+/* ---------------------------------------------------------------- Gaussian random fields
+ * RandomNormal + generate_inhomogeneous_gaussian_random_field (src/random_field.cpp:13-89)
+ * and the spectrum factories (random_field.cpp:212-254). Normals come from a freshly
+ * seeded std::mt19937 + std::normal_distribution<double> on the host. */
+#define CF_SPEC_GAUSS 0           /* exp(-k^2/(2 sigma^2)), P(0)=0; params {sigma} */
+#define CF_SPEC_KM3_GAUSS 1       /* k^-3 exp(-k^2/2), P(0)=0 */
+#define CF_SPEC_SCALE_INV 2       /* scale_invariant_spectrum_3d; params {A_s} */
+#define CF_SPEC_POWER_CUTOFF 3    /* power_law_with_cutoff_given_amplitude_3d; {sigma, k_ast, alpha} */
+#define CF_SPEC_DERIV_POWER_CUTOFF 4 /* to_deriv_spectrum(m, kind 3); {sigma, k_ast, alpha, m} */
+cf_status cf_normals_host(uint32_t seed, int64_t n, double *out_host);
+cf_status cf_spectrum_table_host(int kind, int64_t N, double L, const double *params,
+                                 double *P_by_s_host);
+/* out_dev = GRF from white noise w_dev (N^3, already scaled by 1/N^3 as random_field.cpp:44-46)
+ * and sqrt(P) per s (sqrtP_by_s_dev, cf_spectrum_bins(N) entries); Psi_dev may be NULL.
+ * w_dev is overwritten. */
+cf_status cf_grf(cf_spectrum_plan *plan, double *w_dev, const double *sqrtP_by_s_dev,
+                 const double *Psi_dev, double *out_dev, cf_stream stream);
+/* x *= target / rms(x), rms from a fixed-order device reduction (bit-reproducible). */
+cf_status cf_rescale_rms(double *x_dev, int64_t n, double target, cf_stream stream);
+/* Workload initial condition: phi = GRF(seed, spectrum kind/params) rescaled to rms
+ * (rms <= 0: no rescale), dphi/dt = 0. state_dev holds 2N^3 doubles. Blocking. */
+cf_status cf_init_grf_state(cf_spectrum_plan *plan, uint32_t seed, int kind, const double *params,
+                            double L, double rms, double *state_dev, cf_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CFHIP_H */
