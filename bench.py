@@ -46,7 +46,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first run 3 device-to-device copies of 1 GiB (cf_d2d), the known-size "
+                         "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
 
 
@@ -76,6 +79,14 @@ def main():
         sys.exit(2)
     import torch
     import cfl_amd as cfl
+
+    if args.calib_copy:
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            cfl.check(cfl.lib.cf_d2d(cfl._ptr(b), cfl._ptr(a), 1 << 30, cfl._stream()))
+        torch.cuda.synchronize()
+        del a, b
 
     N, L, m, dt, t0 = args.n, float(args.n), 1.0, 0.05, 1.0
     cosmo = cfl.radiation(1.0, 0.5, 1.0)

@@ -38,7 +38,7 @@ struct SumArgs {
 };
 
 template <int K>
-__global__ void k_scale_sum(double *__restrict__ v0, SumArgs args, int64_t n) {
+__global__ void k_scale_sum(double *v0, SumArgs args, int64_t n) {  // v0 may alias an input (odeint passes x as both)
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {

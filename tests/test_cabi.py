@@ -1,0 +1,93 @@
+"""The C ABI of libcfhip.so without a GPU: it loads, exports every function that
+include/cfhip.h declares, its host-side functions equal the oracle bit for bit,
+and bad arguments are rejected with CF_E_INVALID before any device work."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "cfhip.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cf_[A-Za-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import cfl_amd._lib as L
+    return L
+
+
+def test_library_exports_every_declared_symbol(lib):
+    L = lib.load()
+    names = declared()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # the Python binding declares the same set
+    assert sorted(lib.EXPORTED) == names
+
+
+def test_cosmology_host_functions_equal_oracle(lib, O):
+    L = lib.load()
+    for a1, H1, t1 in ((1.0, 0.5, 1.0), (1.0, 0.05, 10.0), (2.0, 0.3, 0.5)):
+        cg = lib.Cosmology(a1, H1, t1, 1.0)
+        co = O.Cosmo.radiation(a1, H1, t1)
+        for t in (1.0, 3.7, 51.0, 122.5):
+            assert L.cf_cosmo_a(C.byref(cg), t) == O.cosmo_a(co, t)
+            assert L.cf_cosmo_H(C.byref(cg), t) == O.cosmo_H(co, t)
+            assert L.cf_cosmo_eta(C.byref(cg), t) == O.cosmo_eta(co, t)
+    for N, Lb, a in ((64, 64.0, 1.0), (384, 307.2, 2.5), (96, 96.0, 7.1)):
+        assert L.cf_inv_ah_sqr(N, Lb, a) == O.inv_ah_sqr(N, Lb, a)
+
+
+def test_normals_equal_oracle(lib, O):
+    L = lib.load()
+    out = np.empty(10001)
+    lib.check(L.cf_normals_host(77, out.size, out.ctypes.data_as(C.c_void_p)))
+    assert np.array_equal(out, O.normals(77, out.size))
+
+
+@pytest.mark.parametrize("kind,params", [(0, (0.5,)), (1, ()), (2, (4.6e-4,)), (3, (1.0, 1.0, 0.0)),
+                                         (3, (0.02, 1.0, -3.0)), (4, (1.0, 1.0, 0.0, 1.0))])
+def test_spectrum_tables_equal_oracle(lib, O, kind, params):
+    L = lib.load()
+    N, Lb = 64, 51.2
+    prm = np.zeros(8)
+    prm[: len(params)] = params
+    out = np.empty(O.nbins(N))
+    lib.check(L.cf_spectrum_table_host(kind, N, Lb, prm.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p)))
+    assert np.array_equal(out, O.spectrum_table(kind, N, Lb, params))
+
+
+def test_sizes(lib, O):
+    L = lib.load()
+    for N in (4, 64, 96, 512):
+        assert L.cf_spectrum_bins(N) == O.nbins(N)
+    assert L.cf_rk4_work_bytes(512, 0) == 3 * 2 * 512 ** 3 * 8
+    assert L.cf_rk4_work_bytes(512, 1) == 3 * 2 * 512 ** 3 * 4
+    assert L.cf_rk4_work_bytes(7, 0) == 0  # odd N unsupported
+
+
+def test_bad_arguments_rejected_before_device_work(lib):
+    L = lib.load()
+    fake = C.c_void_p(0x1000)
+    assert L.cf_rhs_f64(7, fake, C.c_void_p(0x2000), 1.0, 0.0, 0.0, 1.0, None) == lib.CF_E_INVALID
+    assert b"unsupported" in L.cf_last_error()
+    assert L.cf_rhs_f64(8, fake, fake, 1.0, 0.0, 0.0, 1.0, None) == lib.CF_E_INVALID  # aliased
+    p = lib.KGParams(3, 1.0, 1.0, 0.0, lib.Cosmology(1, 0, 0, 1))
+    assert L.cf_rk4_step_f64(C.byref(p), fake, 0.0, 0.1, fake, None) == lib.CF_E_INVALID
+    p.N, p.L = 8, -1.0
+    assert L.cf_rk4_step_f64(C.byref(p), fake, 0.0, 0.1, fake, None) == lib.CF_E_INVALID
+    assert L.cf_scale_sum(15, fake, None, None, 10, None) == lib.CF_E_INVALID
+    assert L.cf_spectrum_plan_create(10, None) == lib.CF_E_INVALID
+    assert L.cf_spectrum_table_host(9, 8, 1.0, None, C.c_void_p(0)) == lib.CF_E_INVALID
+    with pytest.raises(lib.CfError):
+        lib.check(L.cf_rhs_f64(5, fake, fake, 1, 0, 0, 1, None))
