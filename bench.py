@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # read x and they move 16 passes (256 B); the canonical 272 B is still the
 # figure used for `achieved`.
 BYTES_PER_SITE_STEP = {"f64": 272, "f32c": 152}
-DESIGN_BYTES_PER_SITE_STEP = {"f64": 256, "f32c": 152}
+DESIGN_BYTES_PER_SITE_STEP = {"f64": 256, "f32c": 152, "tb_f64": 32, "tb_f32c": 32}
 
 
 def parse():
@@ -47,6 +47,10 @@ def parse():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--chunk", type=int, default=0, help="a-planes per block of the temporally blocked kernel")
+    ap.add_argument("--save-dir", default="", help="ensemble: where to save the averaged density spectra (.npy)")
+    ap.add_argument("--workload", choices=("headline", "ensemble"), default="headline",
+                    help="headline: N=512 FRW, FP32-comp vs FP64; ensemble: 96 realisations of N=64/96/128")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run 3 device-to-device copies of 1 GiB (cf_d2d), the known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -71,8 +75,72 @@ def cpu_baseline(N, steps):
             "sample": f"oracle FP64 RK4, KG-FRW N={N}^3, {steps} step(s), 1 thread, {el:.1f} s"}
 
 
+def ensemble_workload(args):
+    """96 seeded realisations (seeds 100-131: N=64, 132-163: N=96, 164-195: N=128,
+    L=N), FRW Klein-Gordon with m=1 plus lambda phi^4/4 (lambda=0.1), FP32-comp,
+    P(k) ~ exp(-k^2/(2*0.5^2)), rms 0.5, RK4 dt=0.05, advanced by one persistent
+    kernel per step captured in a hipGraph. Reports aggregate GSite*step/s and the
+    ensemble-averaged density spectrum of each N group."""
+    import numpy as np
+    import torch
+    import cfl_amd as cfl
+    Ns = [64] * 32 + [96] * 32 + [128] * 32
+    seeds = list(range(100, 196))
+    dt, t0 = 0.05, 1.0
+    steps = args.steps
+    ens = cfl.Ensemble(Ns, [float(n) for n in Ns], 1.0, 0.1, cfl.radiation(1.0, 0.5, 1.0), cfl.CF_PREC_F32C)
+    ens.init_grf(seeds, cfl.CF_SPEC_GAUSS, (0.5,), 0.5)
+    tuned_grid = ens.tune(dt, reps=3)
+    plan = {n: ens.get_plan(n)[0] for n in (64, 96, 128)}
+    # capture + first launch outside the timed region (the graph is reused)
+    ens.run(t0, 0, steps, dt, use_graph=True)
+    torch.cuda.synchronize()
+    ens.init_grf(seeds, cfl.CF_SPEC_GAUSS, (0.5,), 0.5)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    ev0.record(stream)
+    ens.run(t0, 0, steps, dt, use_graph=True)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms = ev0.elapsed_time(ev1)
+    rate = ens.sites * steps / (ms * 1e-3) / 1e9
+    t_end = t0 + steps * dt
+    groups = {}
+    for r, n in enumerate(Ns):
+        sp = ens.density_spectrum(r, t_end).cpu().numpy()
+        groups.setdefault(n, []).append(sp)
+    summary = {}
+    for n, lst in groups.items():
+        avg = np.zeros_like(lst[0])
+        for sp in lst:  # fixed order
+            avg += sp
+        avg /= len(lst)
+        if args.save_dir:
+            np.save(os.path.join(args.save_dir, f"ensemble_rho_spectrum_N{n}.npy"), avg)
+        summary[str(n)] = {"bins": int(avg.size), "sum": float(avg.sum()), "first_bins": [float(v) for v in avg[1:5]]}
+    line = {
+        "metric": "aggregate GSite*RK4-step/s, ensemble of 96 uneven-N realisations (FP32-comp, hipGraph)",
+        "value": rate, "unit": "GSite*RK4-step/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": ms / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "storage": "f32 hi + f32 compensation", "data": "synthetic (Gaussian random fields, seeds 100-195)",
+        "config": {"workload": "ensemble 32xN=64 + 32xN=96 + 32xN=128, L=N, FRW KG m=1 lambda=0.1, dt=0.05",
+                   "sites": ens.sites, "tuned_grid": tuned_grid, "tuned_chunks": plan, "graph_wall_s": wall},
+        "ensemble_avg_density_spectra": summary,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
+    if args.workload == "ensemble":
+        if args.gpus != 1:
+            print("bench.py: this build runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        ensemble_workload(args)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "see DESIGN.md for how the lattice would shard", file=sys.stderr)
@@ -98,15 +166,23 @@ def main():
     del plan
 
     results = {}
-    for mode in ("f32c", "f64"):
-        if mode == "f64":
+    for mode in ("f32c", "f64", "tb_f32c", "tb_f64"):
+        if mode.endswith("f64"):
             x = x0.clone()
-            work = cfl.alloc_work(N, cfl.CF_PREC_F64)
-            run = lambda k0, n: cfl.rk4_run_f64(p, x, t0, k0, n, dt, work)  # noqa: E731
+            if mode == "f64":
+                work = cfl.alloc_work(N, cfl.CF_PREC_F64)
+                run = lambda k0, n: cfl.rk4_run_f64(p, x, t0, k0, n, dt, work)  # noqa: E731
+            else:
+                work = cfl.alloc_tb_work(N, cfl.CF_PREC_F64)
+                run = lambda k0, n: cfl.rk4_run_tb_f64(p, x, t0, k0, n, dt, work, args.chunk)  # noqa: E731
         else:
             hi, lo = cfl.split_f64(x0)
-            work = cfl.alloc_work(N, cfl.CF_PREC_F32C)
-            run = lambda k0, n: cfl.rk4_run_f32c(p, hi, lo, t0, k0, n, dt, work)  # noqa: E731
+            if mode == "f32c":
+                work = cfl.alloc_work(N, cfl.CF_PREC_F32C)
+                run = lambda k0, n: cfl.rk4_run_f32c(p, hi, lo, t0, k0, n, dt, work)  # noqa: E731
+            else:
+                work = cfl.alloc_tb_work(N, cfl.CF_PREC_F32C)
+                run = lambda k0, n: cfl.rk4_run_tb_f32c(p, hi, lo, t0, k0, n, dt, work, args.chunk)  # noqa: E731
         run(0, args.warmup)
         torch.cuda.synchronize()
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -120,7 +196,8 @@ def main():
         ms = ev0.elapsed_time(ev1) / args.steps
         rate = N ** 3 / (ms * 1e-3) / 1e9
         results[mode] = {"ms_per_step": ms, "gsite_steps": rate, "wall_s": wall,
-                         "achieved_GBs": BYTES_PER_SITE_STEP[mode] * N ** 3 / (ms * 1e-3) / 1e9}
+                         "achieved_GBs": BYTES_PER_SITE_STEP[mode.replace("tb_", "")] * N ** 3 / (ms * 1e-3) / 1e9,
+                         "moved_GBs": DESIGN_BYTES_PER_SITE_STEP[mode] * N ** 3 / (ms * 1e-3) / 1e9}
         del work
         torch.cuda.synchronize()
 
@@ -146,6 +223,9 @@ def main():
                  "bytes_per_site_step": BYTES_PER_SITE_STEP["f64"],
                  "design_bytes_per_site_step": DESIGN_BYTES_PER_SITE_STEP["f64"]},
         "speedup_f32c_vs_f64": A["gsite_steps"] / B["gsite_steps"],
+        "temporal_blocking": {m: {"value": results["tb_" + m]["gsite_steps"], "ms_per_step": results["tb_" + m]["ms_per_step"],
+                                  "bytes_per_site_step": 32,
+                                  "hbm_GBs_at_32B": results["tb_" + m]["moved_GBs"]} for m in ("f32c", "f64")},
         "roofline": {"bound": "hbm", "kernel": "rk4 step, FP32-comp (4 fused stage kernels)",
                      "achieved": A["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": A["achieved_GBs"] / HBM_PEAK_GBS,

@@ -120,6 +120,24 @@ def rk4_run_f32c(p, hi, lo, t0, k0, n_steps, dt, work, stream=None):
                               _stream(stream)))
 
 
+def tb_work_bytes(N, precision=CF_PREC_F64):
+    return int(lib.cf_rk4_tb_work_bytes(N, precision))
+
+
+def alloc_tb_work(N, precision=CF_PREC_F64, device="cuda"):
+    return torch.empty(tb_work_bytes(N, precision), dtype=torch.uint8, device=device)
+
+
+def rk4_run_tb_f64(p, x, t0, k0, n_steps, dt, work, chunk=0, stream=None):
+    """Temporally blocked RK4 (one kernel per step); same bits as rk4_run_f64."""
+    check(lib.cf_rk4_run_tb_f64(C.byref(p), _ptr(x), t0, k0, n_steps, dt, chunk, _ptr(work), _stream(stream)))
+
+
+def rk4_run_tb_f32c(p, hi, lo, t0, k0, n_steps, dt, work, chunk=0, stream=None):
+    check(lib.cf_rk4_run_tb_f32c(C.byref(p), _ptr(hi), _ptr(lo), t0, k0, n_steps, dt, chunk, _ptr(work),
+                                 _stream(stream)))
+
+
 def split_f64(x, stream=None):
     hi = torch.empty(x.numel(), dtype=torch.float32, device=x.device)
     lo = torch.empty_like(hi)
@@ -235,3 +253,77 @@ def spectrum_table_host(kind, N, L, params=()):
 def rescale_rms(x, target, stream=None):
     check(lib.cf_rescale_rms(_ptr(x), x.numel(), target, _stream(stream)))
     return x
+
+
+# ---------------------------------------------------------------- ensembles
+class Ensemble:
+    """Realisations of uneven N advanced by one persistent kernel per step
+    (cf_ensemble_*); runs can be captured into a hipGraph."""
+
+    def __init__(self, Ns, Ls, m, lam, cosmo, precision=CF_PREC_F32C):
+        n = len(Ns)
+        Na = (C.c_int64 * n)(*[int(v) for v in Ns])
+        La = (C.c_double * n)(*[float(v) for v in Ls])
+        h = C.c_void_p()
+        check(lib.cf_ensemble_create(n, Na, La, m, lam, C.byref(cosmo), precision, C.byref(h)))
+        self.h, self.n, self.Ns, self.Ls, self.precision = h, n, list(Ns), list(Ls), precision
+
+    def close(self):
+        if self.h:
+            lib.cf_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def sites(self):
+        return int(lib.cf_ensemble_sites(self.h))
+
+    def init_grf(self, seeds, kind, params, rms, stream=None):
+        sa = (C.c_uint32 * self.n)(*[int(s) for s in seeds])
+        prm = np.zeros(8)
+        prm[: len(params)] = params
+        check(lib.cf_ensemble_init_grf(self.h, sa, kind, prm.ctypes.data_as(C.c_void_p), rms, _stream(stream)))
+
+    def state_ptrs(self, r):
+        x, lo = C.c_void_p(), C.c_void_p()
+        check(lib.cf_ensemble_state(self.h, r, C.byref(x), C.byref(lo)))
+        return x.value, lo.value
+
+    def state(self, r):
+        """Copy of realisation r's state as a float64 torch tensor (joined for F32C)."""
+        N = self.Ns[r]
+        n = 2 * N ** 3
+        x, lo = self.state_ptrs(r)
+        out = torch.empty(n, dtype=torch.float64, device="cuda")
+        if self.precision == CF_PREC_F32C:
+            check(lib.cf_join_f64(C.c_void_p(x), C.c_void_p(lo), _ptr(out), n, _stream()))
+        else:
+            check(lib.cf_d2d(_ptr(out), C.c_void_p(x), n * 8, _stream()))
+        return out
+
+    def set_plan(self, N=0, chunk=0, grid=0):
+        check(lib.cf_ensemble_set_plan(self.h, N, chunk, grid))
+
+    def get_plan(self, N):
+        c, g = C.c_int(), C.c_int()
+        check(lib.cf_ensemble_get_plan(self.h, N, C.byref(c), C.byref(g)))
+        return c.value, g.value
+
+    def tune(self, dt, reps=3):
+        g = C.c_int()
+        check(lib.cf_ensemble_tune(self.h, dt, reps, C.byref(g)))
+        return g.value
+
+    def run(self, t0, k0, n_steps, dt, use_graph=False, stream=None):
+        check(lib.cf_ensemble_run(self.h, t0, k0, n_steps, dt, int(bool(use_graph)), _stream(stream)))
+
+    def density_spectrum(self, r, t, out=None, stream=None):
+        if out is None:
+            out = torch.empty(nbins(self.Ns[r]), dtype=torch.float64, device="cuda")
+        check(lib.cf_ensemble_density_spectrum(self.h, r, t, _ptr(out), _stream(stream)))
+        return out

@@ -77,6 +77,19 @@ _SIGS = [
     ("cf_rk4_run_f64", st, [C.POINTER(KGParams), vp, dbl, i64, i64, dbl, vp, vp]),
     ("cf_rk4_step_f32c", st, [C.POINTER(KGParams), vp, vp, dbl, dbl, vp, vp]),
     ("cf_rk4_run_f32c", st, [C.POINTER(KGParams), vp, vp, dbl, i64, i64, dbl, vp, vp]),
+    ("cf_rk4_tb_work_bytes", C.c_size_t, [i64, C.c_int]),
+    ("cf_rk4_run_tb_f64", st, [C.POINTER(KGParams), vp, dbl, i64, i64, dbl, C.c_int, vp, vp]),
+    ("cf_rk4_run_tb_f32c", st, [C.POINTER(KGParams), vp, vp, dbl, i64, i64, dbl, C.c_int, vp, vp]),
+    ("cf_ensemble_create", st, [C.c_int, vp, vp, dbl, dbl, C.POINTER(Cosmology), C.c_int, C.POINTER(vp)]),
+    ("cf_ensemble_destroy", st, [vp]),
+    ("cf_ensemble_sites", i64, [vp]),
+    ("cf_ensemble_state", st, [vp, C.c_int, C.POINTER(vp), C.POINTER(vp)]),
+    ("cf_ensemble_init_grf", st, [vp, vp, C.c_int, vp, dbl, vp]),
+    ("cf_ensemble_set_plan", st, [vp, i64, C.c_int, C.c_int]),
+    ("cf_ensemble_get_plan", st, [vp, i64, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("cf_ensemble_tune", st, [vp, dbl, C.c_int, C.POINTER(C.c_int)]),
+    ("cf_ensemble_run", st, [vp, dbl, i64, i64, dbl, C.c_int, vp]),
+    ("cf_ensemble_density_spectrum", st, [vp, C.c_int, dbl, vp, vp]),
     ("cf_split_f64", st, [vp, vp, vp, i64, vp]),
     ("cf_join_f64", st, [vp, vp, vp, i64, vp]),
     ("cf_energy_density_f64", st, [i64, vp, vp, dbl, dbl, dbl, vp]),

@@ -1,0 +1,480 @@
+// cf_ensemble.hip -- ensembles of lattices of uneven N advanced together.
+//
+// All realisations of an ensemble take their RK4 steps in one persistent
+// kernel per step: a fixed grid of 1024-thread blocks walks a table of work
+// units (one output tile of one lattice over a chunk of a-planes, the unit of
+// the temporally blocked step, cf_tb_body.hpp). Units are assigned to blocks on
+// the host by longest-processing-time-first over their iteration counts, so
+// the mix of N = 64 / 96 / 128 lattices finishes together; each block then runs
+// its list in order, with no atomics and no inter-block communication.
+//
+// The step scalars a(t), H(t) of every stage live in a device table, so the
+// kernels of a whole run can be captured once into a hipGraph and replayed
+// (cf_ensemble_run with use_graph). States ping-pong between two buffers per
+// realisation; a step reads buffer p and writes 1-p, which is also what lets the
+// autotuner time candidate plans without touching the state.
+//
+// Each realisation's result equals a single-lattice run of the same steps bit
+// for bit (same per-site arithmetic as cf_rk4_run_tb_* and the stage kernels).
+#include <hipfft/hipfft.h>
+
+#include <algorithm>
+#include <map>
+#include <queue>
+#include <vector>
+
+#include "cf_tb_body.hpp"
+
+namespace cf {
+
+struct EnsReal {
+  int64_t N;
+  double L;
+  void *x[2];      // FP64: state; F32C: hi
+  float *lo[2];    // F32C only
+};
+
+struct EnsUnit {
+  int32_t real;
+  int32_t T;
+  int64_t b0, c0, a0, a1;
+};
+
+struct EnsStepCoef {
+  double a[4], H[4];  // a(t) and H(t) at the four stage times
+};
+
+template <class T, bool F32C, bool LAM>
+__global__ __launch_bounds__(1024) void k_ens_tb(const EnsReal *__restrict__ reals,
+                                                 const int32_t *__restrict__ unit_off,
+                                                 const EnsUnit *__restrict__ units,
+                                                 const EnsStepCoef *__restrict__ steps, int64_t step, int parity,
+                                                 double c_m, double lam, double dt) {
+  __shared__ TbLds<T> lds;
+  const EnsStepCoef sc = steps[step];
+  const int beg = unit_off[blockIdx.x], end = unit_off[blockIdx.x + 1];
+  for (int i = beg; i < end; ++i) {
+    const EnsUnit e = units[i];
+    const EnsReal R = reals[e.real];
+    TbStep k;
+    for (int s = 0; s < 4; ++s) {
+      k.c_h[s] = (-3.0) * sc.H[s];
+      k.inv[s] = 1.0 / ((R.L / R.N) * (R.L / R.N)) / (sc.a[s] * sc.a[s]);  // cf_inv_ah_sqr
+    }
+    k.c_m = c_m;
+    k.lam = lam;
+    k.dh = dt / 2.0;
+    k.dt = dt;
+    k.dt6 = dt / 6.0;
+    k.dt3 = dt / 3.0;
+    TbUnit u;
+    u.N = R.N;
+    u.b0 = e.b0;
+    u.c0 = e.c0;
+    u.a0 = e.a0;
+    u.a1 = e.a1;
+    u.T = e.T;
+    tb_unit<T, F32C, LAM>(lds, u, k, (const T *)R.x[parity], R.lo[parity], (T *)R.x[1 - parity], R.lo[1 - parity]);
+    __syncthreads();  // the next unit rewrites the plane buffers
+  }
+}
+
+}  // namespace cf
+
+using namespace cf;
+
+struct cf_ensemble {
+  int n = 0, prec = CF_PREC_F64;
+  double m = 0, lambda = 0;
+  cf_cosmology cosmo{1, 0, 0, 1};
+  std::vector<EnsReal> reals;
+  std::vector<void *> allocs;
+  EnsReal *d_reals = nullptr;
+  std::map<int64_t, int> chunk;  // planes per unit, per lattice size
+  int grid = 0;
+  int n_units = 0;
+  int32_t *d_unit_off = nullptr;
+  EnsUnit *d_units = nullptr;
+  EnsStepCoef *d_steps = nullptr;
+  int64_t steps_cap = 0;
+  int parity = 0;
+  hipStream_t st = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double g_t0 = 0, g_dt = 0;
+  int64_t g_k0 = -1, g_n = -1;
+  int g_parity = -1;
+  std::map<int64_t, cf_spectrum_plan *> plans;
+  double *scratch = nullptr;  // 3 N_max^3 doubles: a joined state and a density
+};
+
+namespace cf {
+
+static int ens_default_chunk(int64_t N) { return N >= 256 ? 128 : (int)N; }
+
+static void ens_free_plan(cf_ensemble *e) {
+  if (e->d_unit_off) (void)hipFree(e->d_unit_off);
+  if (e->d_units) (void)hipFree(e->d_units);
+  e->d_unit_off = nullptr;
+  e->d_units = nullptr;
+}
+
+static void ens_drop_graph(cf_ensemble *e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  e->exec = nullptr;
+  e->g_k0 = -1;
+}
+
+// Build the unit table for the current chunks and grid, LPT-balanced.
+static cf_status ens_build(cf_ensemble *e) {
+  std::vector<EnsUnit> units;
+  std::vector<int> cost;
+  for (int r = 0; r < e->n; ++r) {
+    const int64_t N = e->reals[r].N;
+    const int T = tb_tile(N);
+    const int64_t nt = cdiv(N, T);
+    const int A = e->chunk[N];
+    for (int64_t a0 = 0; a0 < N; a0 += A)
+      for (int64_t tb = 0; tb < nt; ++tb)
+        for (int64_t tc = 0; tc < nt; ++tc) {
+          EnsUnit u;
+          u.real = r;
+          u.T = T;
+          u.b0 = tb * T;
+          u.c0 = tc * T;
+          u.a0 = a0;
+          u.a1 = std::min<int64_t>(a0 + A, N);
+          units.push_back(u);
+          cost.push_back((int)(u.a1 - u.a0) + 8);
+        }
+  }
+  const int G = std::max(1, std::min<int>(e->grid, (int)units.size()));
+  std::vector<int> order(units.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  typedef std::pair<long, int> Load;  // (load, block)
+  std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+  for (int b = 0; b < G; ++b) heap.push(Load(0, b));
+  std::vector<std::vector<int>> lists(G);
+  for (int i : order) {
+    Load l = heap.top();
+    heap.pop();
+    lists[l.second].push_back(i);
+    heap.push(Load(l.first + cost[i], l.second));
+  }
+  std::vector<int32_t> off(G + 1, 0);
+  std::vector<EnsUnit> flat;
+  for (int b = 0; b < G; ++b) {
+    for (int i : lists[b]) flat.push_back(units[i]);
+    off[b + 1] = (int32_t)flat.size();
+  }
+  ens_free_plan(e);
+  e->grid = G;
+  e->n_units = (int)flat.size();
+  CF_HIP(hipMalloc((void **)&e->d_unit_off, off.size() * sizeof(int32_t)));
+  CF_HIP(hipMalloc((void **)&e->d_units, flat.size() * sizeof(EnsUnit)));
+  CF_HIP(hipMemcpy(e->d_unit_off, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  CF_HIP(hipMemcpy(e->d_units, flat.data(), flat.size() * sizeof(EnsUnit), hipMemcpyHostToDevice));
+  ens_drop_graph(e);
+  return CF_OK;
+}
+
+// Step table for steps k0 .. k0+n-1 at t_k = t0 + k*dt.
+static cf_status ens_fill_steps(cf_ensemble *e, double t0, int64_t k0, int64_t n, double dt) {
+  if (n > e->steps_cap) {
+    if (e->d_steps) CF_HIP(hipFree(e->d_steps));
+    e->d_steps = nullptr;
+    CF_HIP(hipMalloc((void **)&e->d_steps, (size_t)n * sizeof(EnsStepCoef)));
+    e->steps_cap = n;
+  }
+  std::vector<EnsStepCoef> h((size_t)n);
+  for (int64_t s = 0; s < n; ++s) {
+    const double t = t0 + static_cast<double>(k0 + s) * dt;
+    const double dh = dt / 2.0, th = t + dh;
+    const double ts[4] = {t, th, th, t + dt};
+    for (int q = 0; q < 4; ++q) {
+      h[s].a[q] = cf_cosmo_a(&e->cosmo, ts[q]);
+      h[s].H[q] = cf_cosmo_H(&e->cosmo, ts[q]);
+    }
+  }
+  CF_HIP(hipMemcpy(e->d_steps, h.data(), h.size() * sizeof(EnsStepCoef), hipMemcpyHostToDevice));
+  return CF_OK;
+}
+
+static cf_status ens_launch(cf_ensemble *e, int64_t step, int parity, double dt, hipStream_t st) {
+  dim3 grd(e->grid), blk(1024);
+  const double c_m = e->m * e->m;
+  const bool lam = e->lambda != 0.0;
+  if (e->prec == CF_PREC_F32C) {
+    if (lam)
+      hipLaunchKernelGGL((k_ens_tb<float, true, true>), grd, blk, 0, st, e->d_reals, e->d_unit_off, e->d_units,
+                         e->d_steps, step, parity, c_m, e->lambda, dt);
+    else
+      hipLaunchKernelGGL((k_ens_tb<float, true, false>), grd, blk, 0, st, e->d_reals, e->d_unit_off, e->d_units,
+                         e->d_steps, step, parity, c_m, e->lambda, dt);
+  } else {
+    if (lam)
+      hipLaunchKernelGGL((k_ens_tb<double, false, true>), grd, blk, 0, st, e->d_reals, e->d_unit_off, e->d_units,
+                         e->d_steps, step, parity, c_m, e->lambda, dt);
+    else
+      hipLaunchKernelGGL((k_ens_tb<double, false, false>), grd, blk, 0, st, e->d_reals, e->d_unit_off,
+                         e->d_units, e->d_steps, step, parity, c_m, e->lambda, dt);
+  }
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
+// Join the caller's stream into the ensemble's stream and back.
+static cf_status ens_fence(hipStream_t from, hipStream_t to) {
+  hipEvent_t ev;
+  CF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  hipError_t a = hipEventRecord(ev, from);
+  hipError_t b = hipStreamWaitEvent(to, ev, 0);
+  (void)hipEventDestroy(ev);
+  CF_HIP(a);
+  CF_HIP(b);
+  return CF_OK;
+}
+
+// Average time of `reps` launches of step 0 (reads buffer p, writes 1-p).
+static cf_status ens_time(cf_ensemble *e, int reps, double dt, float *ms) {
+  hipEvent_t e0, e1;
+  CF_HIP(hipEventCreate(&e0));
+  CF_HIP(hipEventCreate(&e1));
+  cf_status rc = ens_launch(e, 0, e->parity, dt, e->st);  // warm
+  if (rc == CF_OK && hipEventRecord(e0, e->st) != hipSuccess) rc = CF_E_HIP;
+  for (int i = 0; i < reps && rc == CF_OK; ++i) rc = ens_launch(e, 0, e->parity, dt, e->st);
+  if (rc == CF_OK && hipEventRecord(e1, e->st) != hipSuccess) rc = CF_E_HIP;
+  if (rc == CF_OK && hipEventSynchronize(e1) != hipSuccess) rc = CF_E_HIP;
+  if (rc == CF_OK && hipEventElapsedTime(ms, e0, e1) != hipSuccess) rc = CF_E_HIP;
+  if (rc == CF_OK) *ms /= reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != CF_OK) set_error("ensemble timing failed");
+  return rc;
+}
+
+}  // namespace cf
+
+extern "C" {
+
+cf_status cf_ensemble_destroy(cf_ensemble *e) {
+  if (!e) return CF_OK;
+  ens_drop_graph(e);
+  ens_free_plan(e);
+  if (e->d_steps) (void)hipFree(e->d_steps);
+  if (e->d_reals) (void)hipFree(e->d_reals);
+  for (void *p : e->allocs) (void)hipFree(p);
+  for (auto &kv : e->plans) cf_spectrum_plan_destroy(kv.second);
+  if (e->scratch) (void)hipFree(e->scratch);
+  if (e->st) (void)hipStreamDestroy(e->st);
+  delete e;
+  return CF_OK;
+}
+
+cf_status cf_ensemble_create(int n, const int64_t *N, const double *L, double m, double lambda,
+                             const cf_cosmology *cosmo, int precision, cf_ensemble **out) {
+  CF_REQUIRE(out && N && L && cosmo && n > 0, "cf_ensemble_create: bad arguments");
+  CF_REQUIRE(precision == CF_PREC_F64 || precision == CF_PREC_F32C, "cf_ensemble_create: unknown precision");
+  *out = nullptr;
+  for (int r = 0; r < n; ++r)
+    CF_REQUIRE(valid_N(N[r]) && N[r] <= 1024 && L[r] > 0, "cf_ensemble_create: realisation %d: bad N or L", r);
+  cf_ensemble *e = new cf_ensemble();
+  e->n = n;
+  e->prec = precision;
+  e->m = m;
+  e->lambda = lambda;
+  e->cosmo = *cosmo;
+  e->grid = 1024;
+  auto fail = [&](cf_status s) {
+    cf_ensemble_destroy(e);
+    return s;
+  };
+  if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) {
+    set_error("cf_ensemble_create: stream");
+    return fail(CF_E_HIP);
+  }
+  int64_t nmax = 0;
+  for (int r = 0; r < n; ++r) {
+    EnsReal R{};
+    R.N = N[r];
+    R.L = L[r];
+    nmax = std::max(nmax, N[r]);
+    const size_t cnt = 2 * (size_t)N[r] * N[r] * N[r];
+    for (int q = 0; q < 2; ++q) {
+      void *p = nullptr;
+      const size_t bytes = cnt * (precision == CF_PREC_F32C ? sizeof(float) : sizeof(double));
+      if (hipMalloc(&p, bytes) != hipSuccess) {
+        set_error("cf_ensemble_create: out of device memory");
+        return fail(CF_E_NOMEM);
+      }
+      e->allocs.push_back(p);
+      R.x[q] = p;
+      if (precision == CF_PREC_F32C) {
+        void *l = nullptr;
+        if (hipMalloc(&l, cnt * sizeof(float)) != hipSuccess) {
+          set_error("cf_ensemble_create: out of device memory");
+          return fail(CF_E_NOMEM);
+        }
+        e->allocs.push_back(l);
+        R.lo[q] = (float *)l;
+      }
+    }
+    e->reals.push_back(R);
+    if (!e->chunk.count(N[r])) e->chunk[N[r]] = ens_default_chunk(N[r]);
+  }
+  if (hipMalloc((void **)&e->d_reals, n * sizeof(EnsReal)) != hipSuccess ||
+      hipMemcpy(e->d_reals, e->reals.data(), n * sizeof(EnsReal), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMalloc((void **)&e->scratch, 3 * (size_t)nmax * nmax * nmax * sizeof(double)) != hipSuccess) {
+    set_error("cf_ensemble_create: allocation failed");
+    return fail(CF_E_NOMEM);
+  }
+  cf_status s = ens_build(e);
+  if (s != CF_OK) return fail(s);
+  *out = e;
+  return CF_OK;
+}
+
+int64_t cf_ensemble_sites(const cf_ensemble *e) {
+  int64_t s = 0;
+  if (e)
+    for (const auto &R : e->reals) s += R.N * R.N * R.N;
+  return s;
+}
+
+cf_status cf_ensemble_state(cf_ensemble *e, int r, void **x_dev, float **lo_dev) {
+  CF_REQUIRE(e && r >= 0 && r < e->n && x_dev, "cf_ensemble_state: bad arguments");
+  *x_dev = e->reals[r].x[e->parity];
+  if (lo_dev) *lo_dev = e->prec == CF_PREC_F32C ? e->reals[r].lo[e->parity] : nullptr;
+  return CF_OK;
+}
+
+cf_status cf_ensemble_set_plan(cf_ensemble *e, int64_t N, int chunk, int grid) {
+  CF_REQUIRE(e, "cf_ensemble_set_plan: null ensemble");
+  if (N > 0) {
+    CF_REQUIRE(e->chunk.count(N), "cf_ensemble_set_plan: no lattice of size %lld", (long long)N);
+    e->chunk[N] = (chunk <= 0 || chunk > N) ? (int)N : chunk;
+  }
+  if (grid > 0) e->grid = grid;
+  return ens_build(e);
+}
+
+cf_status cf_ensemble_get_plan(const cf_ensemble *e, int64_t N, int *chunk, int *grid) {
+  CF_REQUIRE(e && chunk && grid, "cf_ensemble_get_plan: bad arguments");
+  auto it = e->chunk.find(N);
+  *chunk = it == e->chunk.end() ? 0 : it->second;
+  *grid = e->grid;
+  return CF_OK;
+}
+
+cf_status cf_ensemble_init_grf(cf_ensemble *e, const uint32_t *seeds, int kind, const double *params, double rms,
+                               cf_stream stream) {
+  CF_REQUIRE(e && seeds, "cf_ensemble_init_grf: bad arguments");
+  for (int r = 0; r < e->n; ++r) {
+    EnsReal &R = e->reals[r];
+    cf_spectrum_plan *&pl = e->plans[R.N];
+    if (!pl) CF_TRY(cf_spectrum_plan_create(R.N, &pl));
+    const int64_t cnt = 2 * R.N * R.N * R.N;
+    double *dst = e->prec == CF_PREC_F64 ? (double *)R.x[e->parity] : e->scratch;
+    CF_TRY(cf_init_grf_state(pl, seeds[r], kind, params, R.L, rms, dst, stream));
+    if (e->prec == CF_PREC_F32C)
+      CF_TRY(cf_split_f64(e->scratch, (float *)R.x[e->parity], R.lo[e->parity], cnt, stream));
+  }
+  CF_HIP(hipStreamSynchronize(S(stream)));
+  return CF_OK;
+}
+
+cf_status cf_ensemble_tune(cf_ensemble *e, double dt, int reps, int *grid_out) {
+  CF_REQUIRE(e, "cf_ensemble_tune: null ensemble");
+  if (reps <= 0) reps = 3;
+  CF_TRY(ens_fill_steps(e, 0.0, 0, 1, dt));
+  const int grids[] = {256, 512, 768, 1024, 2048};
+  float best = 1e30f, ms = 0;
+  int best_g = e->grid;
+  for (int g : grids) {
+    e->grid = g;
+    CF_TRY(ens_build(e));
+    CF_TRY(ens_time(e, reps, dt, &ms));
+    if (ms < best) {
+      best = ms;
+      best_g = g;
+    }
+  }
+  e->grid = best_g;
+  CF_TRY(ens_build(e));
+  for (auto &kv : e->chunk) {
+    const int64_t N = kv.first;
+    int best_a = kv.second;
+    CF_TRY(ens_time(e, reps, dt, &best));
+    for (int A : {8, 16, 32, 64, 128, 256}) {
+      if (A > N || A == best_a) continue;
+      e->chunk[N] = A;
+      CF_TRY(ens_build(e));
+      CF_TRY(ens_time(e, reps, dt, &ms));
+      if (ms < best) {
+        best = ms;
+        best_a = A;
+      }
+    }
+    e->chunk[N] = best_a;
+    CF_TRY(ens_build(e));
+  }
+  if (grid_out) *grid_out = e->grid;
+  return CF_OK;
+}
+
+cf_status cf_ensemble_run(cf_ensemble *e, double t0, int64_t k0, int64_t n_steps, double dt, int use_graph,
+                          cf_stream stream) {
+  CF_REQUIRE(e && n_steps >= 0, "cf_ensemble_run: bad arguments");
+  if (n_steps == 0) return CF_OK;
+  const bool reuse = use_graph && e->exec && e->g_t0 == t0 && e->g_k0 == k0 && e->g_n == n_steps && e->g_dt == dt &&
+                     e->g_parity == e->parity;
+  if (!reuse) CF_TRY(ens_fill_steps(e, t0, k0, n_steps, dt));
+  CF_TRY(ens_fence(S(stream), e->st));
+  if (!use_graph) {
+    for (int64_t s = 0; s < n_steps; ++s) CF_TRY(ens_launch(e, s, e->parity ^ (int)(s & 1), dt, e->st));
+  } else {
+    if (!reuse) {
+      ens_drop_graph(e);
+      hipGraph_t g;
+      CF_HIP(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
+      cf_status rc = CF_OK;
+      for (int64_t s = 0; s < n_steps && rc == CF_OK; ++s) rc = ens_launch(e, s, e->parity ^ (int)(s & 1), dt, e->st);
+      hipError_t ce = hipStreamEndCapture(e->st, &g);
+      if (rc != CF_OK) return rc;
+      CF_HIP(ce);
+      hipError_t ie = hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      CF_HIP(ie);
+      e->g_t0 = t0;
+      e->g_k0 = k0;
+      e->g_n = n_steps;
+      e->g_dt = dt;
+      e->g_parity = e->parity;
+    }
+    CF_HIP(hipGraphLaunch(e->exec, e->st));
+  }
+  CF_TRY(ens_fence(e->st, S(stream)));
+  e->parity ^= (int)(n_steps & 1);
+  return CF_OK;
+}
+
+// Energy-density power spectrum of realisation r at time t (observer.hpp:102-108),
+// into bins_dev (cf_spectrum_bins(N) doubles).
+cf_status cf_ensemble_density_spectrum(cf_ensemble *e, int r, double t, double *bins_dev, cf_stream stream) {
+  CF_REQUIRE(e && r >= 0 && r < e->n && bins_dev, "cf_ensemble_density_spectrum: bad arguments");
+  EnsReal &R = e->reals[r];
+  cf_spectrum_plan *&pl = e->plans[R.N];
+  if (!pl) CF_TRY(cf_spectrum_plan_create(R.N, &pl));
+  const int64_t n3 = R.N * R.N * R.N;
+  const double *x = (const double *)R.x[e->parity];
+  if (e->prec == CF_PREC_F32C) {
+    CF_TRY(cf_join_f64((const float *)R.x[e->parity], R.lo[e->parity], e->scratch, 2 * n3, stream));
+    x = e->scratch;
+  }
+  double *rho = e->scratch + 2 * n3;  // after the joined state (scratch holds 3 N_max^3)
+  const double a_t = cf_cosmo_a(&e->cosmo, t);
+  CF_TRY(cf_energy_density_f64(R.N, x, rho, e->m, e->lambda, cf_inv_ah_sqr(R.N, R.L, a_t), stream));
+  return cf_power_spectrum(pl, rho, bins_dev, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+// cf_rk4_tb.hip -- temporally blocked classic RK4 on one lattice: all four
+// stages of a step in one kernel, so the state is read once and written once
+// per step (the step body, its geometry and its arithmetic are described in
+// cf_tb_body.hpp). One block per work unit (output tile x a-chunk).
+//
+// HBM per site-step: FP64 read x, write x' = 32 B; FP32-comp read hi, lo, write
+// hi', lo' = 32 B (plus halo and warm-up re-reads, mostly served by L2).
+// The output goes to a second buffer (neighbouring blocks still read the old
+// state), so runs ping-pong between two state arrays.
+#include <cmath>
+
+#include "cf_tb_body.hpp"
+
+namespace cf {
+
+struct TbGeom {
+  int64_t N;
+  int T;      // output tile edge
+  int nt;     // tiles per dimension
+  int A;      // planes per chunk
+  int n_ac;   // chunks
+  int total;  // nt * nt * n_ac
+};
+
+template <class T, bool F32C, bool LAM>
+__global__ __launch_bounds__(1024) void k_rk4_tb(TbGeom g, TbStep k, const T *__restrict__ xin,
+                                                 const float *__restrict__ lin, T *__restrict__ xout,
+                                                 float *__restrict__ lout) {
+  __shared__ TbLds<T> lds;
+  // Blocks go to the 8 XCDs round robin; consecutive logical tiles are mapped
+  // to one XCD so that neighbouring tiles' halos meet in its L2.
+  int bid = blockIdx.x;
+  if ((g.total & 7) == 0) bid = (bid & 7) * (g.total >> 3) + (bid >> 3);
+  const int tc = bid % g.nt;
+  const int rest = bid / g.nt;
+  const int tb = rest % g.nt;
+  const int ac = rest / g.nt;
+  TbUnit u;
+  u.N = g.N;
+  u.T = g.T;
+  u.b0 = (int64_t)tb * g.T;
+  u.c0 = (int64_t)tc * g.T;
+  u.a0 = (int64_t)ac * g.A;
+  u.a1 = (u.a0 + g.A < g.N) ? u.a0 + g.A : g.N;
+  tb_unit<T, F32C, LAM>(lds, u, k, xin, lin, xout, lout);
+}
+
+static TbGeom tb_geom(int64_t N, int A) {
+  TbGeom g;
+  g.N = N;
+  g.T = tb_tile(N);
+  g.nt = (int)cdiv(N, g.T);
+  g.A = (A <= 0 || A > N) ? (int)N : A;
+  g.n_ac = (int)cdiv(N, g.A);
+  g.total = g.nt * g.nt * g.n_ac;
+  return g;
+}
+
+static int tb_default_chunk(int64_t N) { return N >= 256 ? 256 : (int)N; }
+
+template <class T, bool F32C>
+static cf_status tb_launch(const TbGeom &g, const TbStep &k, const T *xin, const float *lin, T *xout,
+                           float *lout, hipStream_t st) {
+  if (k.lam != 0.0)
+    hipLaunchKernelGGL((k_rk4_tb<T, F32C, true>), dim3(g.total), dim3(1024), 0, st, g, k, xin, lin, xout, lout);
+  else
+    hipLaunchKernelGGL((k_rk4_tb<T, F32C, false>), dim3(g.total), dim3(1024), 0, st, g, k, xin, lin, xout, lout);
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
+static cf_status tb_check(const cf_kg_params *p) {
+  CF_REQUIRE(p, "null cf_kg_params");
+  CF_REQUIRE(valid_N(p->N), "unsupported lattice size N=%lld (even, 4..2048)", (long long)p->N);
+  CF_REQUIRE(p->L > 0.0, "box size L must be positive");
+  return CF_OK;
+}
+
+}  // namespace cf
+
+using namespace cf;
+
+extern "C" {
+
+size_t cf_rk4_tb_work_bytes(int64_t N, int precision) {
+  if (!valid_N(N)) return 0;
+  const size_t n = 2 * (size_t)N * N * N;
+  return precision == CF_PREC_F32C ? 2 * n * sizeof(float) : n * sizeof(double);
+}
+
+cf_status cf_rk4_run_tb_f64(const cf_kg_params *p, double *x, double t0, int64_t k0, int64_t n_steps, double dt,
+                            int chunk, void *work, cf_stream stream) {
+  CF_TRY(tb_check(p));
+  CF_REQUIRE(x && work && (void *)x != work, "cf_rk4_run_tb_f64: null or aliased buffer");
+  CF_REQUIRE(n_steps >= 0, "cf_rk4_run_tb_f64: negative step count");
+  const TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N));
+  double *bufs[2] = {x, (double *)work};
+  for (int64_t s = 0; s < n_steps; ++s) {
+    const double t = t0 + static_cast<double>(k0 + s) * dt;
+    CF_TRY((tb_launch<double, false>(g, tb_step_host(p, t, dt), bufs[s & 1], nullptr, bufs[(s + 1) & 1], nullptr,
+                                      S(stream))));
+  }
+  if (n_steps & 1) {
+    const size_t bytes = 2 * (size_t)p->N * p->N * p->N * sizeof(double);
+    CF_HIP(hipMemcpyAsync(x, work, bytes, hipMemcpyDeviceToDevice, S(stream)));
+  }
+  return CF_OK;
+}
+
+cf_status cf_rk4_run_tb_f32c(const cf_kg_params *p, float *hi, float *lo, double t0, int64_t k0, int64_t n_steps,
+                             double dt, int chunk, void *work, cf_stream stream) {
+  CF_TRY(tb_check(p));
+  CF_REQUIRE(hi && lo && work && hi != lo, "cf_rk4_run_tb_f32c: null or aliased buffer");
+  CF_REQUIRE(n_steps >= 0, "cf_rk4_run_tb_f32c: negative step count");
+  const TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N));
+  const size_t n = 2 * (size_t)p->N * p->N * p->N;
+  float *his[2] = {hi, (float *)work};
+  float *los[2] = {lo, (float *)work + n};
+  for (int64_t s = 0; s < n_steps; ++s) {
+    const double t = t0 + static_cast<double>(k0 + s) * dt;
+    CF_TRY((tb_launch<float, true>(g, tb_step_host(p, t, dt), his[s & 1], los[s & 1], his[(s + 1) & 1],
+                                    los[(s + 1) & 1], S(stream))));
+  }
+  if (n_steps & 1) {
+    CF_HIP(hipMemcpyAsync(hi, his[1], n * sizeof(float), hipMemcpyDeviceToDevice, S(stream)));
+    CF_HIP(hipMemcpyAsync(lo, los[1], n * sizeof(float), hipMemcpyDeviceToDevice, S(stream)));
+  }
+  return CF_OK;
+}
+
+}  // extern "C"

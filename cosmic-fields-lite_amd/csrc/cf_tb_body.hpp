@@ -1,0 +1,233 @@
+// cf_tb_body.hpp -- the temporally blocked RK4 step of one work unit (an output
+// tile of one lattice over a chunk of a-planes), shared by the single-lattice
+// kernel (cf_rk4_tb.hip) and the persistent ensemble kernel (cf_ensemble.hip).
+//
+// Geometry. A block is 32 x 32 threads over a 32 x 32 (b, c) region of the
+// lattice: an output tile of T x T sites (T = 24, or N for small lattices) plus
+// a halo of 4 sites on every side, one per stencil stage. The block marches
+// along a through a chunk of planes [a0, a1). Each thread owns one (b, c)
+// column and keeps, in registers, the short history of planes every stage
+// needs along a:
+//
+//   iteration j loads x at plane P = a0 - 4 + j, then computes
+//     stage 1 at P-1 (needs x at P-2..P)       -> y1, acc
+//     stage 2 at P-2 (needs y1 at P-3..P-1)    -> y2, acc
+//     stage 3 at P-3 (needs y2 at P-4..P-2)    -> y3, acc
+//     stage 4 at P-4 (needs y3 at P-5..P-3)    -> new state at P-4 (stored for j >= 8)
+//
+// so a chunk of A planes takes A + 8 iterations. The in-plane neighbours
+// (b+-1, c+-1) of each stage input come from plane buffers in LDS. The four
+// stage inputs of an iteration are all known when it starts (the a+1 neighbour
+// each stage needs from the current iteration is in registers), so they are
+// written together and one barrier per iteration suffices. The region shrinks by
+// one site per stage; only the inner T x T columns store.
+//
+// Arithmetic. Per site and stage the operations, their order and their
+// rounding are those of the fused stage kernels (cf_stencil.hip): FP64 keeps
+// acc = 1*x + dt/6 k1 + dt/3 k2 + dt/3 k3 and ends with acc + dt/6 k4; the
+// compensated FP32 mode rounds y and the increment D to float after every
+// stage and folds (hi + lo) + (D + dt/6 k4) in double. The results are
+// therefore bit-identical to the four-kernel path.
+#pragma once
+
+#include "cf_common.hpp"
+
+namespace cf {
+
+constexpr int TBR = 32;  // region edge (threads per row, rows per block)
+constexpr int TBH = 4;   // halo: one site per stencil stage
+
+// Scalars of one step: per stage (-3H) and inv_ah_sqr at the stage time, and
+// the step's constants.
+struct TbStep {
+  double c_h[4], inv[4];
+  double c_m, lam, dh, dt, dt6, dt3;
+};
+
+// One work unit: lattice size, output tile origin and edge, plane chunk.
+struct TbUnit {
+  int64_t N;
+  int64_t b0, c0, a0, a1;
+  int T;
+};
+
+__device__ __forceinline__ double tb_rhs(double phi, double pa1, double pam, double pb1, double pbm, double pc1,
+                                         double pcm, double pi, double c_h, double c_m, double lam, double inv,
+                                         bool has_lam) {
+  double v = c_h * pi - c_m * phi;
+  if (has_lam) v = v - ((lam * phi) * phi) * phi;
+  double r = v + inv * ((((-6.0 * phi + pa1) + pam) + pb1) + pbm);
+  return r + inv * (pc1 + pcm);
+}
+
+// float rounding of the compensated mode (identity for FP64)
+template <bool F32C>
+__device__ __forceinline__ double rnd(double v) {
+  if constexpr (F32C) return (double)(float)v;
+  else return v;
+}
+
+// Two sets of four plane buffers: iteration j writes the four stage inputs
+// (all known when it starts: they are the previous iteration's outputs) into set
+// j&1, then one barrier. Set j&1 was last read in iteration j-2, which every
+// thread finished before the barrier of iteration j-1. A caller running several
+// units in a row must barrier between them.
+template <class T>
+using TbLds = T[2][4][TBR][TBR];
+
+template <class T, bool F32C, bool LAM>
+__device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const TbStep &k, const T *__restrict__ xin,
+                                        const float *__restrict__ lin, T *__restrict__ xout,
+                                        float *__restrict__ lout) {
+  const int tid = threadIdx.x;
+  const int r = tid >> 5, cc = tid & 31;
+  const int64_t N = u.N, NN = N * N, N3 = NN * N;
+  int64_t b = (u.b0 - TBH + r) % N;
+  if (b < 0) b += N;
+  int64_t c = (u.c0 - TBH + cc) % N;
+  if (c < 0) c += N;
+  const int64_t col = b * N + c;
+  const bool out_ok = r >= TBH && r < TBH + u.T && u.b0 + (r - TBH) < N && cc >= TBH && cc < TBH + u.T &&
+                      u.c0 + (cc - TBH) < N;
+  const int rm = r > 0 ? r - 1 : 0, rp = r < TBR - 1 ? r + 1 : TBR - 1;
+  const int cm = cc > 0 ? cc - 1 : 0, cp = cc < TBR - 1 ? cc + 1 : TBR - 1;
+
+  // plane offsets (times N^2) advanced by one plane per iteration, wrapping at N
+  auto wrap_plane = [&](int64_t P) -> int64_t {
+    int64_t q = P % N;
+    if (q < 0) q += N;
+    return q * NN;
+  };
+  auto next_plane = [&](int64_t off) -> int64_t {
+    off += NN;
+    return off == N3 ? 0 : off;
+  };
+
+  double xf0 = 0, xf1 = 0, xf2 = 0, xf3 = 0, xf4 = 0;  // x phi at P, P-1, ..., P-4
+  double xp0 = 0, xp1 = 0, xp2 = 0, xp3 = 0, xp4 = 0;  // x pi
+  double y1f1 = 0, y1f2 = 0, y1p1 = 0;                  // y1 phi at P-2, P-3 (start of iteration); pi at P-2
+  double y2f1 = 0, y2f2 = 0, y2p1 = 0;                  // y2 phi at P-3, P-4; pi at P-3
+  double y3f1 = 0, y3f2 = 0, y3p1 = 0;                  // y3 phi at P-4, P-5; pi at P-4
+  double af1 = 0, af2 = 0, af3 = 0, ap1 = 0, ap2 = 0, ap3 = 0;  // acc at P-2, P-3, P-4
+
+  const int iters = (int)(u.a1 - u.a0) + 8;
+  // two planes of prefetch: pf0 = x at P, pf1 = x at P+1
+  int64_t off_ld = wrap_plane(u.a0 - TBH);
+  double pf0f = (double)xin[off_ld + col], pf0p = (double)xin[N3 + off_ld + col];
+  off_ld = next_plane(off_ld);
+  double pf1f = (double)xin[off_ld + col], pf1p = (double)xin[N3 + off_ld + col];
+  int64_t off_st = wrap_plane(u.a0 - TBH - 4);  // plane P-4
+
+  for (int j = 0; j < iters; ++j) {
+    xf0 = pf0f;
+    xp0 = pf0p;
+    pf0f = pf1f;
+    pf0p = pf1p;
+    off_ld = next_plane(off_ld);  // plane P+2
+    pf1f = (double)xin[off_ld + col];
+    pf1p = (double)xin[N3 + off_ld + col];
+    const bool store = (j >= 8) && out_ok;
+    double lo_f = 0.0, lo_p = 0.0;
+    if constexpr (F32C) {
+      if (store) {
+        lo_f = (double)lin[off_st + col];
+        lo_p = (double)lin[N3 + off_st + col];
+      }
+    }
+
+    T(*buf)[TBR][TBR] = lds[j & 1];
+    buf[0][r][cc] = (T)xf1;
+    buf[1][r][cc] = (T)y1f1;
+    buf[2][r][cc] = (T)y2f1;
+    buf[3][r][cc] = (T)y3f1;
+    __syncthreads();
+
+    // ---- stage 1 at P-1: k1 = f(x)
+    double kf = xp1;
+    double kp = tb_rhs(xf1, xf0, xf2, (double)buf[0][rp][cc], (double)buf[0][rm][cc], (double)buf[0][r][cp],
+                       (double)buf[0][r][cm], xp1, k.c_h[0], k.c_m, k.lam, k.inv[0], LAM);
+    const double y1f0 = rnd<F32C>(xf1 + k.dh * kf);
+    const double y1p0 = rnd<F32C>(xp1 + k.dh * kp);
+    double af0, ap0;
+    if constexpr (F32C) {
+      af0 = rnd<true>(k.dt6 * kf);
+      ap0 = rnd<true>(k.dt6 * kp);
+    } else {
+      af0 = xf1 + k.dt6 * kf;
+      ap0 = xp1 + k.dt6 * kp;
+    }
+
+    // ---- stage 2 at P-2: k2 = f(y1)
+    kf = y1p1;
+    kp = tb_rhs(y1f1, y1f0, y1f2, (double)buf[1][rp][cc], (double)buf[1][rm][cc], (double)buf[1][r][cp],
+                (double)buf[1][r][cm], y1p1, k.c_h[1], k.c_m, k.lam, k.inv[1], LAM);
+    const double y2f0 = rnd<F32C>(xf2 + k.dh * kf);
+    const double y2p0 = rnd<F32C>(xp2 + k.dh * kp);
+    af1 = rnd<F32C>(af1 + k.dt3 * kf);
+    ap1 = rnd<F32C>(ap1 + k.dt3 * kp);
+
+    // ---- stage 3 at P-3: k3 = f(y2)
+    kf = y2p1;
+    kp = tb_rhs(y2f1, y2f0, y2f2, (double)buf[2][rp][cc], (double)buf[2][rm][cc], (double)buf[2][r][cp],
+                (double)buf[2][r][cm], y2p1, k.c_h[2], k.c_m, k.lam, k.inv[2], LAM);
+    const double y3f0 = rnd<F32C>(xf3 + k.dt * kf);
+    const double y3p0 = rnd<F32C>(xp3 + k.dt * kp);
+    af2 = rnd<F32C>(af2 + k.dt3 * kf);
+    ap2 = rnd<F32C>(ap2 + k.dt3 * kp);
+
+    // ---- stage 4 at P-4: k4 = f(y3); new state
+    kf = y3p1;
+    kp = tb_rhs(y3f1, y3f0, y3f2, (double)buf[3][rp][cc], (double)buf[3][rm][cc], (double)buf[3][r][cp],
+                (double)buf[3][r][cm], y3p1, k.c_h[3], k.c_m, k.lam, k.inv[3], LAM);
+    if (store) {
+      if constexpr (F32C) {
+        const double s0 = (xf4 + lo_f) + (af3 + k.dt6 * kf);
+        const double s1 = (xp4 + lo_p) + (ap3 + k.dt6 * kp);
+        const float h0 = (float)s0, h1 = (float)s1;
+        xout[off_st + col] = h0;
+        xout[N3 + off_st + col] = h1;
+        lout[off_st + col] = (float)(s0 - (double)h0);
+        lout[N3 + off_st + col] = (float)(s1 - (double)h1);
+      } else {
+        xout[off_st + col] = af3 + k.dt6 * kf;
+        xout[N3 + off_st + col] = ap3 + k.dt6 * kp;
+      }
+    }
+    off_st = next_plane(off_st);
+
+    // ---- advance the histories by one plane
+    xf4 = xf3; xf3 = xf2; xf2 = xf1; xf1 = xf0;
+    xp4 = xp3; xp3 = xp2; xp2 = xp1; xp1 = xp0;
+    y1f2 = y1f1; y1f1 = y1f0; y1p1 = y1p0;
+    y2f2 = y2f1; y2f1 = y2f0; y2p1 = y2p0;
+    y3f2 = y3f1; y3f1 = y3f0; y3p1 = y3p0;
+    af3 = af2; af2 = af1; af1 = af0;
+    ap3 = ap2; ap2 = ap1; ap1 = ap0;
+  }
+}
+
+// Output tile edge for an N^3 lattice (whole plane when N < 24).
+inline int tb_tile(int64_t N) { return (N < TBR - 2 * TBH) ? (int)N : TBR - 2 * TBH; }
+
+// Host: the step's scalars (runge_kutta4_classic.hpp:97-123 stage times; the
+// RHS scalars as equations.cpp:75-80). inv_ah_sqr follows cf_inv_ah_sqr.
+inline TbStep tb_step_host(const cf_kg_params *p, double t, double dt) {
+  TbStep k;
+  const double dh = dt / 2.0;
+  const double th = t + dh;
+  const double ts[4] = {t, th, th, t + dt};
+  for (int s = 0; s < 4; ++s) {
+    const double a_t = cf_cosmo_a(&p->cosmo, ts[s]);
+    k.c_h[s] = (-3.0) * cf_cosmo_H(&p->cosmo, ts[s]);
+    k.inv[s] = cf_inv_ah_sqr(p->N, p->L, a_t);
+  }
+  k.c_m = p->m * p->m;
+  k.lam = p->lambda;
+  k.dh = dh;
+  k.dt = dt;
+  k.dt6 = dt / 6.0;
+  k.dt3 = dt / 3.0;
+  return k;
+}
+
+}  // namespace cf

@@ -123,9 +123,50 @@ cf_status cf_rk4_step_f32c(const cf_kg_params *p, float *hi_dev, float *lo_dev, 
                            void *work_dev, cf_stream stream);
 cf_status cf_rk4_run_f32c(const cf_kg_params *p, float *hi_dev, float *lo_dev, double t0, int64_t k0,
                           int64_t n_steps, double dt, void *work_dev, cf_stream stream);
+/* Temporally blocked variant: all four stages of a step in one kernel (state read
+ * once and written once per step, 32 B/site-step), same arithmetic and bits as
+ * cf_rk4_run_f64 / cf_rk4_run_f32c. Steps ping-pong between the state and
+ * work_dev (cf_rk4_tb_work_bytes); the result is always left in the state.
+ * chunk = planes per block along a (<= 0: default). */
+size_t cf_rk4_tb_work_bytes(int64_t N, int precision);
+cf_status cf_rk4_run_tb_f64(const cf_kg_params *p, double *x_dev, double t0, int64_t k0, int64_t n_steps,
+                            double dt, int chunk, void *work_dev, cf_stream stream);
+cf_status cf_rk4_run_tb_f32c(const cf_kg_params *p, float *hi_dev, float *lo_dev, double t0, int64_t k0,
+                             int64_t n_steps, double dt, int chunk, void *work_dev, cf_stream stream);
+
 /* Split a double state into hi/lo floats and back (x = hi + lo). */
 cf_status cf_split_f64(const double *x_dev, float *hi_dev, float *lo_dev, int64_t n, cf_stream stream);
 cf_status cf_join_f64(const float *hi_dev, const float *lo_dev, double *x_dev, int64_t n, cf_stream stream);
+
+/* ---------------------------------------------------------------- ensembles
+ * Seeded realisations of uneven N advanced together (no reference counterpart:
+ * the reference runs one lattice per process, src/main.cpp:82-196). One persistent
+ * kernel per step walks all realisations' work units; a run can be captured once
+ * into a hipGraph and replayed. Each realisation's result equals a single-lattice
+ * run of the same steps bit for bit. All realisations share m, lambda, cosmology. */
+typedef struct cf_ensemble cf_ensemble;
+cf_status cf_ensemble_create(int n, const int64_t *N, const double *L, double m, double lambda,
+                             const cf_cosmology *cosmo, int precision, cf_ensemble **e);
+cf_status cf_ensemble_destroy(cf_ensemble *e);
+int64_t cf_ensemble_sites(const cf_ensemble *e); /* sum of N^3 */
+/* Current state of realisation r: x (FP64) or hi (F32C) and lo (F32C, else NULL).
+ * Valid until the next run (states ping-pong). */
+cf_status cf_ensemble_state(cf_ensemble *e, int r, void **x_dev, float **lo_dev);
+/* Initial conditions: realisation r gets the GRF of seeds[r] (cf_init_grf_state). */
+cf_status cf_ensemble_init_grf(cf_ensemble *e, const uint32_t *seeds, int kind, const double *params, double rms,
+                               cf_stream stream);
+/* Work plan: planes per unit for lattices of size N (N <= 0: keep) and the grid. */
+cf_status cf_ensemble_set_plan(cf_ensemble *e, int64_t N, int chunk, int grid);
+cf_status cf_ensemble_get_plan(const cf_ensemble *e, int64_t N, int *chunk, int *grid);
+/* Autotuner: times candidate grids, then candidate chunks per N, on the device
+ * (a step reads one buffer and writes the other, so the state is untouched), and
+ * keeps the fastest plan. Blocking. */
+cf_status cf_ensemble_tune(cf_ensemble *e, double dt, int reps, int *grid_out);
+/* n_steps RK4 steps at t_k = t0 + (k0 + k) dt; use_graph: capture (or reuse) a hipGraph. */
+cf_status cf_ensemble_run(cf_ensemble *e, double t0, int64_t k0, int64_t n_steps, double dt, int use_graph,
+                          cf_stream stream);
+/* Energy-density power spectrum of realisation r at time t into bins_dev. */
+cf_status cf_ensemble_density_spectrum(cf_ensemble *e, int r, double t, double *bins_dev, cf_stream stream);
 
 /* ---------------------------------------------------------------- energy density
  * Equation::compute_energy_density (equations.cpp:106-138; energy_density_kernel,

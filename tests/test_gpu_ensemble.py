@@ -1,0 +1,130 @@
+"""Ensembles of uneven-N lattices in one persistent launch per step.
+
+Every realisation must equal a single-lattice run of the same steps bit for bit
+(so the output does not depend on the batch composition), graph replays must
+equal the eager run (replayed several times with eager work in between), and
+the autotuner's plan must not change any bit.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from test_gpu_parity import assert_bitwise, host
+
+pytestmark = pytest.mark.gpu
+
+FRW = dict(a1=1.0, H1=0.5, t1=1.0)
+NS = [16, 24, 32, 16, 40, 24]
+
+
+def raw_state(cfl, ens, r):
+    """hi, lo (F32C) or x (F64) of realisation r as fresh torch tensors."""
+    N = ens.Ns[r]
+    n = 2 * N ** 3
+    x, lo = ens.state_ptrs(r)
+    if ens.precision == cfl.CF_PREC_F32C:
+        hi_t = torch.empty(n, dtype=torch.float32, device="cuda")
+        lo_t = torch.empty_like(hi_t)
+        cfl.check(cfl.lib.cf_d2d(cfl._ptr(hi_t), C.c_void_p(x), n * 4, cfl._stream()))
+        cfl.check(cfl.lib.cf_d2d(cfl._ptr(lo_t), C.c_void_p(lo), n * 4, cfl._stream()))
+        return hi_t, lo_t
+    x_t = torch.empty(n, dtype=torch.float64, device="cuda")
+    cfl.check(cfl.lib.cf_d2d(cfl._ptr(x_t), C.c_void_p(x), n * 8, cfl._stream()))
+    return (x_t,)
+
+
+def make(cfl, prec, lam=0.1):
+    ens = cfl.Ensemble(NS, [1.0 * n for n in NS], 1.0, lam, cfl.radiation(**FRW), prec)
+    ens.init_grf(list(range(100, 100 + len(NS))), cfl.CF_SPEC_GAUSS, (0.5,), 0.5)
+    return ens
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_ensemble_equals_single_lattice_runs(cfl, prec):
+    print(f"case ensemble prec={prec}", flush=True)
+    ens = make(cfl, prec)
+    t0, k0, steps, dt = 1.0, 2, 3, 0.05
+    init = [raw_state(cfl, ens, r) for r in range(ens.n)]
+    ens.run(t0, k0, steps, dt)
+    for r, N in enumerate(NS):
+        p = cfl.kg_params(N, float(N), 1.0, 0.1, cfl.radiation(**FRW))
+        if prec == cfl.CF_PREC_F32C:
+            hi, lo = init[r]
+            cfl.rk4_run_f32c(p, hi, lo, t0, k0, steps, dt, cfl.alloc_work(N, prec))
+            ghi, glo = raw_state(cfl, ens, r)
+            assert_bitwise(host(ghi), host(hi), f"hi r={r}")
+            assert_bitwise(host(glo), host(lo), f"lo r={r}")
+        else:
+            (x,) = init[r]
+            cfl.rk4_run_f64(p, x, t0, k0, steps, dt, cfl.alloc_work(N, prec))
+            (gx,) = raw_state(cfl, ens, r)
+            assert_bitwise(host(gx), host(x), f"x r={r}")
+    ens.close()
+
+
+def test_ensemble_oracle_small(cfl, O):
+    """One realisation against the oracle, from the same initial state."""
+    ens = make(cfl, 0)
+    x0 = host(ens.state(1))
+    ens.run(1.0, 0, 2, 0.05)
+    eq = O.Equation(24, 24.0, 1.0, 0.1, O.Cosmo.radiation(**FRW))
+    x = x0.copy()
+    eq.rk4_run(x, 1.0, 0.05, 2)
+    assert_bitwise(host(ens.state(1)), x, "ensemble r=1 vs oracle")
+    ens.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_graph_replays_equal_eager(cfl, prec):
+    print(f"case graph prec={prec}", flush=True)
+    seeds = list(range(100, 100 + len(NS)))
+    ens = make(cfl, prec)
+    ens.run(1.0, 0, 4, 0.05, use_graph=False)
+    torch.cuda.synchronize()
+    eager = [host(ens.state(r)) for r in range(ens.n)]
+    for rep in range(3):
+        # eager work between replays: re-initialise (GRF + FFT + rescale) and sync
+        ens.init_grf(seeds, cfl.CF_SPEC_GAUSS, (0.5,), 0.5)
+        torch.cuda.synchronize()
+        ens.run(1.0, 0, 4, 0.05, use_graph=True)
+        torch.cuda.synchronize()
+        for r in range(ens.n):
+            assert_bitwise(host(ens.state(r)), eager[r], f"replay {rep} r={r}")
+    ens.close()
+
+
+def test_autotuner_keeps_bits(cfl):
+    ens = make(cfl, 1)
+    x0 = [raw_state(cfl, ens, r) for r in range(ens.n)]
+    g = ens.tune(0.05, reps=2)
+    assert g > 0
+    # tuning did not touch the state
+    for r in range(ens.n):
+        for a, b in zip(raw_state(cfl, ens, r), x0[r]):
+            assert torch.equal(a, b)
+    ens.run(1.0, 0, 2, 0.05)
+    after_tuned = [host(ens.state(r)) for r in range(ens.n)]
+    ens.close()
+    ens2 = make(cfl, 1)
+    ens2.set_plan(0, 0, 97)  # an arbitrary different plan
+    ens2.run(1.0, 0, 2, 0.05)
+    for r in range(ens2.n):
+        assert_bitwise(host(ens2.state(r)), after_tuned[r], f"plan-independence r={r}")
+    ens2.close()
+
+
+def test_density_spectrum(cfl, O):
+    ens = make(cfl, 1)
+    ens.run(1.0, 0, 2, 0.05)
+    t = 1.1
+    for r in (0, 4):
+        N = NS[r]
+        x = host(ens.state(r))
+        eq = O.Equation(N, float(N), 1.0, 0.1, O.Cosmo.radiation(**FRW))
+        ref = O.power_spectrum(N, eq.energy_density(x, t))
+        g = host(ens.density_spectrum(r, t))
+        assert np.all(g[ref == 0] == 0)
+        assert np.all(np.abs(g - ref) <= 1e-10 * ref + 1e-18 * ref.max())
+    ens.close()

@@ -1,0 +1,68 @@
+"""Temporally blocked RK4 (one kernel per step) against the oracle and the
+four-kernel path: bit-identical states for every lattice size class (N smaller
+than the tile, partial edge tiles, several a-chunks, odd step counts that end
+with the ping-pong copy) and both precisions."""
+import numpy as np
+import pytest
+import torch
+
+from test_gpu_parity import assert_bitwise, dev, host, rand_state
+
+pytestmark = pytest.mark.gpu
+
+FRW = dict(a1=1.0, H1=0.5, t1=1.0)
+
+
+@pytest.mark.parametrize("N,lam,steps,chunk", [(8, 0.0, 3, 0), (16, 0.1, 2, 0), (24, 0.0, 1, 0),
+                                               (40, 0.1, 3, 7), (64, 0.0, 2, 16), (96, 0.1, 3, 0)])
+def test_tb_f64_bitwise_oracle(cfl, O, N, lam, steps, chunk):
+    print(f"case tb f64 N={N} lam={lam} steps={steps} chunk={chunk}", flush=True)
+    L = 1.25 * N
+    eq = O.Equation(N, L, 1.0, lam, O.Cosmo.radiation(**FRW))
+    x = rand_state(N, 21, amp=0.5)
+    xd = dev(x)
+    p = cfl.kg_params(N, L, 1.0, lam, cfl.radiation(**FRW))
+    cfl.rk4_run_tb_f64(p, xd, 1.0, 3, steps, 0.05, cfl.alloc_tb_work(N), chunk=chunk)
+    for k in range(steps):
+        eq.rk4_step(x, 1.0 + (3 + k) * 0.05, 0.05)
+    assert_bitwise(host(xd), x, "tb state")
+
+
+@pytest.mark.parametrize("N,lam,steps", [(16, 0.0, 3), (64, 0.1, 2), (96, 0.0, 1)])
+def test_tb_f32c_bitwise_oracle(cfl, O, N, lam, steps):
+    print(f"case tb f32c N={N} lam={lam} steps={steps}", flush=True)
+    L = float(N)
+    eq = O.Equation(N, L, 1.0, lam, O.Cosmo.radiation(**FRW))
+    x = rand_state(N, 22, amp=0.5)
+    hi = x.astype(np.float32)
+    lo = (x - hi.astype(np.float64)).astype(np.float32)
+    hd, ld = dev(hi.copy()), dev(lo.copy())
+    p = cfl.kg_params(N, L, 1.0, lam, cfl.radiation(**FRW))
+    cfl.rk4_run_tb_f32c(p, hd, ld, 1.0, 0, steps, 0.05, cfl.alloc_tb_work(N, cfl.CF_PREC_F32C))
+    for k in range(steps):
+        eq.rk4_step_f32c(hi, lo, 1.0 + k * 0.05, 0.05)
+    assert_bitwise(host(hd), hi, "tb f32c hi")
+    assert_bitwise(host(ld), lo, "tb f32c lo")
+
+
+@pytest.mark.parametrize("N", [256, 512])
+def test_tb_equals_stage_kernels_large(cfl, N):
+    """Size-independent property at full size: both schedules give the same bits."""
+    print(f"case tb vs stages N={N}", flush=True)
+    plan = cfl.SpectrumPlan(N)
+    x0 = plan.init_grf_state(2, cfl.CF_SPEC_GAUSS, (0.25,), float(N), 1.0)
+    del plan
+    p = cfl.kg_params(N, float(N), 1.0, 0.0, cfl.radiation(**FRW))
+    a = x0.clone()
+    cfl.rk4_run_f64(p, a, 1.0, 0, 2, 0.05, cfl.alloc_work(N))
+    b = x0.clone()
+    cfl.rk4_run_tb_f64(p, b, 1.0, 0, 2, 0.05, cfl.alloc_tb_work(N))
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    del a, b
+    hi, lo = cfl.split_f64(x0)
+    h2, l2 = hi.clone(), lo.clone()
+    cfl.rk4_run_f32c(p, hi, lo, 1.0, 0, 2, 0.05, cfl.alloc_work(N, cfl.CF_PREC_F32C))
+    cfl.rk4_run_tb_f32c(p, h2, l2, 1.0, 0, 2, 0.05, cfl.alloc_tb_work(N, cfl.CF_PREC_F32C))
+    torch.cuda.synchronize()
+    assert torch.equal(hi, h2) and torch.equal(lo, l2)
