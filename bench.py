@@ -49,8 +49,10 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--chunk", type=int, default=0, help="a-planes per block of the temporally blocked kernel")
     ap.add_argument("--save-dir", default="", help="ensemble: where to save the averaged density spectra (.npy)")
-    ap.add_argument("--workload", choices=("headline", "ensemble"), default="headline",
-                    help="headline: N=512 FRW, FP32-comp vs FP64; ensemble: 96 realisations of N=64/96/128")
+    ap.add_argument("--workload", choices=("headline", "ensemble", "precision"), default="headline",
+                    help="headline: N=512 FRW, FP32-comp vs FP64; ensemble: 96 realisations of N=64/96/128; "
+                         "precision: run A (FP32-comp) vs run B (FP64) errors every 100 steps")
+    ap.add_argument("--report-every", type=int, default=100, help="precision: steps between error reports")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run 3 device-to-device copies of 1 GiB (cf_d2d), the known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -133,8 +135,81 @@ def ensemble_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def precision_workload(args):
+    """Run A (FP32 hi + FP32 compensation, FP64 arithmetic) and run B (FP64) of the
+    N=512 FRW workload from the same initial state (seed 2, P(k) ~ exp(-k^2/(2*0.25^2)),
+    rms 1, dt 0.05) for --steps steps (1000 for the full study). Every
+    --report-every steps: relative L2 error of phi and dphi/dt of A against B, and
+    the largest relative error over non-empty bins of the field (mode) spectrum
+    and of the energy-density spectrum. Both runs use the temporally blocked step
+    kernels (bit-identical to the per-stage kernels)."""
+    import torch
+    import cfl_amd as cfl
+    N, L, m, dt, t0 = args.n, float(args.n), 1.0, 0.05, 1.0
+    cosmo = cfl.radiation(1.0, 0.5, 1.0)
+    p = cfl.kg_params(N, L, m, 0.0, cosmo)
+    plan = cfl.SpectrumPlan(N)
+    xB = plan.init_grf_state(2, cfl.CF_SPEC_GAUSS, (0.25,), L, 1.0)
+    hi, lo = cfl.split_f64(xB)
+    wA = cfl.alloc_tb_work(N, cfl.CF_PREC_F32C)
+    wB = cfl.alloc_tb_work(N, cfl.CF_PREC_F64)
+    rho = torch.empty(N ** 3, dtype=torch.float64, device="cuda")
+    n3 = N ** 3
+
+    def spectra(x, t):
+        a_t = cfl.cosmo_a(cosmo, t)
+        field = plan.mode_power_spectrum(x, L, m, a_t).clone()
+        cfl.energy_density_f64(N, x, rho, m, 0.0, cfl.inv_ah_sqr(N, L, a_t))
+        return field, plan.power_spectrum(rho).clone()
+
+    def rel_bins(a, b):
+        nz = b != 0
+        return float(((a[nz] - b[nz]).abs() / b[nz].abs()).max())
+
+    rows = []
+    done = 0
+    w0 = time.perf_counter()
+    while done < args.steps:
+        k = min(args.report_every, args.steps - done)
+        cfl.rk4_run_tb_f32c(p, hi, lo, t0, done, k, dt, wA)
+        cfl.rk4_run_tb_f64(p, xB, t0, done, k, dt, wB)
+        done += k
+        t = t0 + done * dt
+        xA = cfl.join_f64(hi, lo)
+        row = {"step": done, "t": t}
+        for name, sl in (("phi", slice(0, n3)), ("dphi", slice(n3, 2 * n3))):
+            row[f"state_rel_l2_{name}"] = float(torch.linalg.norm(xA[sl] - xB[sl]) / torch.linalg.norm(xB[sl]))
+        fA, rA = spectra(xA, t)
+        fB, rB = spectra(xB, t)
+        row["field_spectrum_max_rel"] = rel_bins(fA, fB)
+        row["rho_spectrum_max_rel"] = rel_bins(rA, rB)
+        del xA
+        rows.append(row)
+        print(json.dumps({"progress": row}), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    last = rows[-1]
+    line = {
+        "metric": "relative error of FP32-comp (run A) vs FP64 (run B), N=512^3 FRW",
+        "value": max(last["field_spectrum_max_rel"], last["rho_spectrum_max_rel"]),
+        "unit": "max relative spectrum error at the last step", "higher_is_better": False,
+        "n_gpus": 1, "steps": args.steps, "dtype": "f64", "storage": "A: f32 hi + f32 compensation; B: f64",
+        "data": "synthetic (Gaussian random field, seed 2)",
+        "config": {"workload": f"KG-FRW radiation era N={N}^3 L={L:g} m=1 dt=0.05 RK4, A vs B every "
+                               f"{args.report_every} steps", "wall_s": time.perf_counter() - w0},
+        "gate_final_spectrum_1e-6": bool(max(last["field_spectrum_max_rel"], last["rho_spectrum_max_rel"]) <= 1e-6),
+        "per_report": rows,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
+    if args.workload == "precision":
+        if args.gpus != 1:
+            print("bench.py: this build runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        precision_workload(args)
+        return
     if args.workload == "ensemble":
         if args.gpus != 1:
             print("bench.py: this build runs on one GPU only", file=sys.stderr)

@@ -13,8 +13,9 @@ import torch
 
 from . import _lib
 from ._lib import (CF_PREC_F32C, CF_PREC_F64, CF_SPEC_DERIV_POWER_CUTOFF, CF_SPEC_GAUSS,  # noqa: F401
-                   CF_SPEC_KM3_GAUSS, CF_SPEC_POWER_CUTOFF, CF_SPEC_SCALE_INV, CfError, Cosmology,
-                   KGParams, check)
+                   CF_SPEC_KM3_GAUSS, CF_SPEC_POWER_CUTOFF, CF_SPEC_SCALE_INV,
+                   CF_VAR_APPROX_COMOVING_CURVATURE, CF_VAR_COMOVING_CURVATURE, CF_VAR_FIXED_CURVATURE,
+                   CF_VAR_SQRT_POTENTIAL, CfError, Cosmology, KGParams, check)
 
 lib = _lib.load()
 LIB_PATH = _lib.LIB_PATH
@@ -327,3 +328,72 @@ class Ensemble:
             out = torch.empty(nbins(self.Ns[r]), dtype=torch.float64, device="cuda")
         check(lib.cf_ensemble_density_spectrum(self.h, r, t, _ptr(out), _stream(stream)))
         return out
+
+
+# ---------------------------------------------------------------- equation variants
+class Variant:
+    """A curvature / potential variant of the lattice equation (cf_variant_*):
+    kind CF_VAR_FIXED_CURVATURE (field = Psi, N^3), CF_VAR_SQRT_POTENTIAL (f_a),
+    CF_VAR_COMOVING_CURVATURE (field = R_fft, the half spectrum of the comoving
+    curvature) or CF_VAR_APPROX_COMOVING_CURVATURE (R_fft and the coarse grid M)."""
+
+    def __init__(self, p, kind, field=None, f_a=1.0, M=0, stream=None):
+        h = C.c_void_p()
+        check(lib.cf_variant_create(C.byref(p), kind, f_a, M, _ptr(field), C.byref(h), _stream(stream)))
+        self.h, self.p, self.kind, self.M = h, p, kind, M
+        self.N = p.N
+        self._work = None
+
+    def close(self):
+        if self.h:
+            lib.cf_variant_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def rhs(self, x, t, out=None, stream=None):
+        if out is None:
+            out = torch.empty_like(x)
+        check(lib.cf_variant_rhs(self.h, _ptr(x), _ptr(out), t, _stream(stream)))
+        return out
+
+    def _w(self):
+        if self._work is None:
+            self._work = alloc_work(self.N, CF_PREC_F64)
+        return self._work
+
+    def rk4_step(self, x, t, dt, stream=None):
+        check(lib.cf_variant_rk4_step(self.h, _ptr(x), t, dt, _ptr(self._w()), _stream(stream)))
+
+    def rk4_run(self, x, t0, k0, n_steps, dt, stream=None):
+        check(lib.cf_variant_rk4_run(self.h, _ptr(x), t0, k0, n_steps, dt, _ptr(self._w()), _stream(stream)))
+
+    def energy_density(self, x, t, out=None, stream=None):
+        if out is None:
+            out = torch.empty(self.N ** 3, dtype=torch.float64, device="cuda")
+        check(lib.cf_variant_energy_density(self.h, _ptr(x), _ptr(out), t, _stream(stream)))
+        return out
+
+    def psi(self, t, stream=None):
+        G = self.M if self.kind == CF_VAR_APPROX_COMOVING_CURVATURE else self.N
+        ps = torch.empty(G ** 3, dtype=torch.float64, device="cuda")
+        dps = torch.empty_like(ps)
+        check(lib.cf_variant_psi(self.h, t, _ptr(ps), _ptr(dps), _stream(stream)))
+        return ps, dps
+
+
+def comoving_psi_fft(G, R_fft, factor, H, stream=None):
+    ps = torch.empty_like(R_fft)
+    dps = torch.empty_like(R_fft)
+    check(lib.cf_comoving_psi_fft(G, _ptr(R_fft), _ptr(ps), _ptr(dps), factor, H, _stream(stream)))
+    return ps, dps
+
+
+def cutoff_fouriers(N, M, fk, stream=None):
+    out = torch.empty(2 * M * M * (M // 2 + 1), dtype=torch.float64, device="cuda")
+    check(lib.cf_cutoff_fouriers(N, M, _ptr(fk), _ptr(out), _stream(stream)))
+    return out

@@ -21,6 +21,11 @@ CF_E_UNSUPPORTED = 5
 CF_PREC_F64 = 0
 CF_PREC_F32C = 1
 
+CF_VAR_FIXED_CURVATURE = 1
+CF_VAR_SQRT_POTENTIAL = 2
+CF_VAR_COMOVING_CURVATURE = 3
+CF_VAR_APPROX_COMOVING_CURVATURE = 4
+
 CF_SPEC_GAUSS = 0
 CF_SPEC_KM3_GAUSS = 1
 CF_SPEC_SCALE_INV = 2
@@ -90,6 +95,21 @@ _SIGS = [
     ("cf_ensemble_tune", st, [vp, dbl, C.c_int, C.POINTER(C.c_int)]),
     ("cf_ensemble_run", st, [vp, dbl, i64, i64, dbl, C.c_int, vp]),
     ("cf_ensemble_density_spectrum", st, [vp, C.c_int, dbl, vp, vp]),
+    ("cf_adjoint_work_bytes", C.c_size_t, [i64, i64, i64]),
+    ("cf_loss_spectrum_adjoint", st, [vp, C.POINTER(KGParams), vp, dbl, vp, vp, vp, C.POINTER(dbl),
+                                      C.POINTER(dbl), vp]),
+    ("cf_rk4_step_adjoint_f64", st, [C.POINTER(KGParams), vp, dbl, dbl, vp, vp, vp, vp]),
+    ("cf_adjoint_spectrum_loss", st, [vp, C.POINTER(KGParams), vp, dbl, dbl, i64, i64, vp, vp, C.POINTER(dbl),
+                                      C.POINTER(dbl), vp, vp]),
+    ("cf_variant_create", st, [C.POINTER(KGParams), C.c_int, dbl, i64, vp, C.POINTER(vp), vp]),
+    ("cf_variant_destroy", None, [vp]),
+    ("cf_variant_rhs", st, [vp, vp, vp, dbl, vp]),
+    ("cf_variant_rk4_step", st, [vp, vp, dbl, dbl, vp, vp]),
+    ("cf_variant_rk4_run", st, [vp, vp, dbl, i64, i64, dbl, vp, vp]),
+    ("cf_variant_energy_density", st, [vp, vp, vp, dbl, vp]),
+    ("cf_variant_psi", st, [vp, dbl, vp, vp, vp]),
+    ("cf_comoving_psi_fft", st, [i64, vp, vp, vp, dbl, dbl, vp]),
+    ("cf_cutoff_fouriers", st, [i64, i64, vp, vp, vp]),
     ("cf_split_f64", st, [vp, vp, vp, i64, vp]),
     ("cf_join_f64", st, [vp, vp, vp, i64, vp]),
     ("cf_energy_density_f64", st, [i64, vp, vp, dbl, dbl, dbl, vp]),

@@ -89,7 +89,77 @@ struct HipKG {
   std::shared_ptr<hip_vector<unsigned char>> work_ = std::make_shared<hip_vector<unsigned char>>();
 };
 
+// Owns a cf_variant handle (a snapshot of the workspace's Psi or R_fft).
+class VariantHandle {
+ public:
+  VariantHandle(const WorkspaceGeneric<hip_vector<double>> &w, int kind) {
+    cf_kg_params p = KGCore::params(w, 0.0, true);
+    const double *field = nullptr;
+    if (kind == CF_VAR_FIXED_CURVATURE) field = w.Psi.data();
+    if (kind == CF_VAR_COMOVING_CURVATURE || kind == CF_VAR_APPROX_COMOVING_CURVATURE) field = w.R_fft.data();
+    check(cf_variant_create(&p, kind, w.f_a, w.M, field, &v_, default_stream()), "cf_variant_create");
+  }
+  ~VariantHandle() { cf_variant_destroy(v_); }
+  VariantHandle(const VariantHandle &) = delete;
+  VariantHandle &operator=(const VariantHandle &) = delete;
+  cf_variant get() const { return v_; }
+
+ private:
+  cf_variant v_ = nullptr;
+};
+
+// The curvature / potential equations (cf_variant_*). The workspace's Psi
+// (fixed curvature) or R_fft (comoving curvature) is copied when the equation is
+// constructed; call refresh() after changing it.
+template <int KIND>
+struct HipVariant {
+  typedef hip_vector<double> Vector;
+  typedef Vector State;
+  typedef WorkspaceGeneric<Vector> Workspace;
+  Workspace &workspace;
+
+  explicit HipVariant(Workspace &workspace_)
+      : workspace(workspace_), h_(std::make_shared<VariantHandle>(workspace_, KIND)) {}
+
+  void refresh() { h_ = std::make_shared<VariantHandle>(workspace, KIND); }
+
+  void operator()(const State &x, State &dxdt, const double t) {
+    if (dxdt.size() != x.size()) dxdt.resize(x.size());
+    check(cf_variant_rhs(h_->get(), x.data(), dxdt.data(), t, default_stream()), "Equation::operator()");
+  }
+
+  // The reference takes Workspace& for the curvature variants (equations_cuda.cuh:88).
+  static Vector compute_energy_density(Workspace &w, const double t) {
+    VariantHandle h(w, KIND);
+    Vector rho((std::size_t)(w.N * w.N * w.N));
+    check(cf_variant_energy_density(h.get(), w.state.data(), rho.data(), t, default_stream()),
+          "compute_energy_density");
+    return rho;
+  }
+
+  void rk4_step(State &x, double t, double dt) {
+    const std::size_t wb = cf_rk4_work_bytes(workspace.N, CF_PREC_F64);
+    if (work_->size() != wb) work_->resize(wb);
+    check(cf_variant_rk4_step(h_->get(), x.data(), t, dt, work_->data(), default_stream()), "rk4_step");
+  }
+
+ private:
+  std::shared_ptr<VariantHandle> h_;
+  std::shared_ptr<hip_vector<unsigned char>> work_ = std::make_shared<hip_vector<unsigned char>>();
+};
+
 }  // namespace detail
+
+// CudaSqrtPotentialEquationInFRW (equations_cuda.cu:83-106, 610-642): f_a from the workspace.
+typedef detail::HipVariant<CF_VAR_SQRT_POTENTIAL> HipSqrtPotentialEquationInFRW;
+// CudaFixedCurvatureEquationInFRW (equations_cuda.cu:114-137, 645-675): workspace.Psi.
+typedef detail::HipVariant<CF_VAR_FIXED_CURVATURE> HipFixedCurvatureEquationInFRW;
+// ComovingCurvatureEquationInFRW / CudaComovingCurvatureEquationInFRW
+// (equations.cpp:141-322; equations_cuda.cu:678-761): workspace.R_fft.
+typedef detail::HipVariant<CF_VAR_COMOVING_CURVATURE> HipComovingCurvatureEquationInFRW;
+// CudaApproximateComovingCurvatureEquationInFRW (equations_cuda.cu:764-815):
+// workspace.R_fft and workspace.M.
+typedef detail::HipVariant<CF_VAR_APPROX_COMOVING_CURVATURE> HipApproximateComovingCurvatureEquationInFRW;
 
 // KleinGordonEquation (equations.cpp:8-36): Minkowski, h = L/N.
 typedef detail::HipKG<false, false> HipKleinGordonEquation;

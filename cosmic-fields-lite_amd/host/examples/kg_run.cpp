@@ -7,7 +7,7 @@
 //   stepper:  classic (odeint_lite::runge_kutta4_classic over hip_operations)
 //             fused   (cfl::hip_runge_kutta4: fused stage kernels)
 //             f32c    (cfl::hip_runge_kutta4_f32c: FP32-compensated storage)
-//   equation: kg | frw | lambda
+//   equation: kg | frw | lambda | sqrt (sqrt potential; the lambda argument is f_a)
 // Writes state0.dat (initial state), the observer's spectra, t_list.dat and state.dat.
 #include <cstdio>
 #include <cstdlib>
@@ -27,6 +27,7 @@ struct RunParam {
   double t_end;
   double t_interval;
   double delta_t;
+  double f_a;
 };
 
 template <class Equation>
@@ -44,6 +45,9 @@ static int run(const std::string &dir, const std::string &stepper, const RunPara
   } else if (stepper == "fused") {
     steps = cfl::odeint_lite::integrate_const(cfl::hip_runge_kutta4<>(), eqn, workspace.state, param.t_start,
                                               param.t_end, param.delta_t, observer);
+  } else if constexpr (!requires { eqn.params(); }) {
+    std::fprintf(stderr, "stepper %s is not available for this equation\n", stepper.c_str());
+    return 2;
   } else if (stepper == "f32c") {
     // Compensated state; the observer sees the joined double state.
     cfl::compensated_state xs = cfl::to_compensated(workspace.state);
@@ -74,12 +78,13 @@ int main(int argc, char **argv) {
   }
   const std::string dir = argv[1], stepper = argv[2], eq = argv[3];
   RunParam p{atoll(argv[4]), atof(argv[5]), atof(argv[6]), atof(argv[7]), atof(argv[8]), atof(argv[9]),
-             atof(argv[10]), atof(argv[11]), atof(argv[12]), atof(argv[13]), atof(argv[14])};
+             atof(argv[10]), atof(argv[11]), atof(argv[12]), atof(argv[13]), atof(argv[14]), atof(argv[7])};
   cfl::grf_initializer init{(uint32_t)atol(argv[15]), atoi(argv[16]), {atof(argv[17]), 0, 0, 0}, atof(argv[18])};
   try {
     if (eq == "kg") return run<cfl::HipKleinGordonEquation>(dir, stepper, p, init);
     if (eq == "frw") return run<cfl::HipKleinGordonEquationInFRW>(dir, stepper, p, init);
     if (eq == "lambda") return run<cfl::HipLambdaEquationInFRW>(dir, stepper, p, init);
+    if (eq == "sqrt") return run<cfl::HipSqrtPotentialEquationInFRW>(dir, stepper, p, init);
   } catch (const std::exception &e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

@@ -33,6 +33,7 @@ typedef int32_t cf_status;
 #define CF_E_UNSUPPORTED 5 /* valid request this build does not implement */
 
 typedef void *cf_stream; /* a hipStream_t; NULL is the default stream */
+typedef struct cf_spectrum_plan cf_spectrum_plan; /* FFT plans + bin table of one N (below) */
 
 /* ---------------------------------------------------------------- status */
 const char *cf_last_error(void);
@@ -168,17 +169,88 @@ cf_status cf_ensemble_run(cf_ensemble *e, double t0, int64_t k0, int64_t n_steps
 /* Energy-density power spectrum of realisation r at time t into bins_dev. */
 cf_status cf_ensemble_density_spectrum(cf_ensemble *e, int r, double t, double *bins_dev, cf_stream stream);
 
+/* ---------------------------------------------------------------- adjoint
+ * Discrete adjoint of the RK4 evolution and of the spectral loss
+ * L = sum_s w_s P_rho(s) (P_rho: compute_power_spectrum of the energy density,
+ * fdm3d.cpp:7-34, at the final time; w_s given per bin in weights_dev). No
+ * reference counterpart (the reference has no gradients); the forward pieces are
+ * the ones above, so gradients are of exactly the computed quantities. */
+size_t cf_adjoint_work_bytes(int64_t N, int64_t n_steps, int64_t ckpt);
+/* L, dL/dx (gx_dev, 2N^3) and the direct dL/dm at state x and time t.
+ * scratch_dev: N^3 doubles. Blocking. */
+cf_status cf_loss_spectrum_adjoint(cf_spectrum_plan *plan, const cf_kg_params *p, const double *x_dev, double t,
+                                   const double *weights_dev, double *gx_dev, double *scratch_dev, double *loss,
+                                   double *dldm, cf_stream stream);
+/* One step back: lam_dev holds dL/dx_{n+1} on entry, dL/dx_n on return, for the
+ * step taken from x_dev at time t; dL/dm contributions go to part_dev (2048
+ * doubles, summed by the caller in order). work_dev: 7 * 2N^3 doubles. */
+cf_status cf_rk4_step_adjoint_f64(const cf_kg_params *p, const double *x_dev, double t, double dt, double *lam_dev,
+                                  double *part_dev, void *work_dev, cf_stream stream);
+/* Forward n_steps from x0 (checkpoint every ckpt steps), loss at t0 + n_steps dt,
+ * backward: grad_x0_dev = dL/dx0 (2N^3), *loss, *dldm (all of m's paths). Blocking.
+ * work_dev: cf_adjoint_work_bytes(N, n_steps, ckpt). */
+cf_status cf_adjoint_spectrum_loss(cf_spectrum_plan *plan, const cf_kg_params *p, const double *x0_dev, double t0,
+                                   double dt, int64_t n_steps, int64_t ckpt, const double *weights_dev,
+                                   double *grad_x0_dev, double *loss, double *dldm, void *work_dev,
+                                   cf_stream stream);
+
 /* ---------------------------------------------------------------- energy density
  * Equation::compute_energy_density (equations.cpp:106-138; energy_density_kernel,
  * equations_cuda.cu:332-354 via :537-553 and :591-607): rho (N^3) from the state. */
 cf_status cf_energy_density_f64(int64_t N, const double *x_dev, double *rho_dev, double m,
                                 double lambda, double inv_ah2, cf_stream stream);
 
+/* ---------------------------------------------------------------- equation variants
+ * The curvature / potential equations, FP64:
+ *   CF_VAR_FIXED_CURVATURE      CudaFixedCurvatureEquationInFRW (equations_cuda.cu:114-137,
+ *                               645-675); field = Psi (N^3, real space), copied.
+ *   CF_VAR_SQRT_POTENTIAL       CudaSqrtPotentialEquationInFRW (equations_cuda.cu:83-106,
+ *                               610-642); V = m^2 f_a^2 (sqrt(1 + phi^2/f_a^2) - 1).
+ *   CF_VAR_COMOVING_CURVATURE   ComovingCurvatureEquationInFRW (equations.cpp:141-322) /
+ *                               CudaComovingCurvatureEquationInFRW (equations_cuda.cu:678-761);
+ *                               field = R_fft, the unnormalised R2C of the comoving
+ *                               curvature (N*N*(N/2+1) complex), copied. Psi, dPsi/dt
+ *                               are rebuilt at every distinct stage time.
+ *   CF_VAR_APPROX_COMOVING_CURVATURE  CudaApproximateComovingCurvatureEquationInFRW
+ *                               (equations_cuda.cu:299-329, 764-815): Psi on an M^3 grid
+ *                               from the cut-off spectrum (compute_cutoff_fouriers,
+ *                               fdm3d_cuda.cu:117-141, 206-219), Taylor-expanded exponentials;
+ *                               M must divide N. field = R_fft of the N^3 lattice.
+ * The comoving variants need N (and M) <= 1024. A handle keeps a small cache of Psi
+ * fields keyed by time: use it from one stream at a time. */
+#define CF_VAR_FIXED_CURVATURE 1
+#define CF_VAR_SQRT_POTENTIAL 2
+#define CF_VAR_COMOVING_CURVATURE 3
+#define CF_VAR_APPROX_COMOVING_CURVATURE 4
+typedef struct cf_variant_s *cf_variant;
+/* p->lambda is ignored. f_a: sqrt potential only. M: approximate variant only. Blocking. */
+cf_status cf_variant_create(const cf_kg_params *p, int kind, double f_a, int64_t M, const double *field_dev,
+                            cf_variant *out, cf_stream stream);
+void cf_variant_destroy(cf_variant v);
+/* Equation::operator()(x, dxdt, t): the odeint system call. */
+cf_status cf_variant_rhs(cf_variant v, const double *x_dev, double *dxdt_dev, double t, cf_stream stream);
+/* One / n fused classic RK4 steps (as cf_rk4_step_f64 / cf_rk4_run_f64);
+ * work_dev: cf_rk4_work_bytes(N, CF_PREC_F64). */
+cf_status cf_variant_rk4_step(cf_variant v, double *x_dev, double t, double dt, void *work_dev, cf_stream stream);
+cf_status cf_variant_rk4_run(cf_variant v, double *x_dev, double t0, int64_t k0, int64_t n_steps, double dt,
+                             void *work_dev, cf_stream stream);
+/* Equation::compute_energy_density (equations_cuda.cu:422-473, equations.cpp:262-322). */
+cf_status cf_variant_energy_density(cf_variant v, const double *x_dev, double *rho_dev, double t,
+                                    cf_stream stream);
+/* Comoving variants: Psi(t) and dPsi/dt(t) in real space (G^3 each, G = N or M). */
+cf_status cf_variant_psi(cf_variant v, double t, double *psi_dev, double *dpsi_dev, cf_stream stream);
+/* compute_Comoving_Curvature_Psi_dPsidt_fft (equations.cpp:141-183): half spectra of a G^3
+ * grid; factor = (2 pi/L) eta(t)/sqrt(3). */
+cf_status cf_comoving_psi_fft(int64_t G, const double *R_fft_dev, double *psi_fft_dev, double *dpsi_fft_dev,
+                              double factor, double H, cf_stream stream);
+/* compute_cutoff_fouriers (fdm3d.cpp:105-130; fdm3d_cuda.cu:117-141): half spectrum of N^3
+ * -> half spectrum of M^3, divided by (N/M)^3. */
+cf_status cf_cutoff_fouriers(int64_t N, int64_t M, const double *in_dev, double *out_dev, cf_stream stream);
+
 /* ---------------------------------------------------------------- FFT and spectra
  * A spectrum plan owns the hipFFT D2Z/Z2D plans of an N^3 lattice with one shared
  * work area (cufftWrapperNoBatching, src/cuda_wrapper.cu:202-255), scratch buffers,
  * and the bin table used by the deterministic binning kernels. */
-typedef struct cf_spectrum_plan cf_spectrum_plan;
 cf_status cf_spectrum_plan_create(int64_t N, cf_spectrum_plan **plan);
 cf_status cf_spectrum_plan_destroy(cf_spectrum_plan *plan);
 int64_t cf_spectrum_bins(int64_t N); /* 3*(N/2)^2 + 1 */

@@ -693,3 +693,142 @@ extern "C" double orc_ci_pade(double x) {
                        3.800404484365274e-7 * pow(x, 6) + 1.1376490214488613e-9 * pow(x, 8) +
                        2.297129602871981e-12 * pow(x, 10) + 2.510407760855278e-15 * pow(x, 12));
 }
+
+// ---------------------------------------------------------------------------
+// Curvature / potential variants
+// ---------------------------------------------------------------------------
+// The 7-point sum of the CUDA kernels, all six neighbours in one expression:
+// ((((((-6 phi + a+) + a-) + b+) + b-) + c+) + c-)   (equations_cuda.cu:28-34)
+static inline double stencil7(i64 N, const double *x, i64 a, i64 b, i64 c) {
+  return (((((-6.0 * x[IDX(N, a, b, c)] + x[IDX(N, (a + 1) % N, b, c)]) + x[IDX(N, (a + N - 1) % N, b, c)]) +
+            x[IDX(N, a, (b + 1) % N, c)]) +
+           x[IDX(N, a, (b + N - 1) % N, c)]) +
+          x[IDX(N, a, b, (c + 1) % N)]) +
+         x[IDX(N, a, b, (c + N - 1) % N)];
+}
+
+// Fixed_Curvature_FRW_equation_kernel (equations_cuda.cu:114-137); no CPU counterpart.
+extern "C" void orc_rhs_fixed_curvature(i64 N, const double *x, double *dxdt, const double *Psi, double m,
+                                        double H, double inv) {
+  const i64 N3 = N * N * N;
+  parallel_for(N, [&](i64 a) {
+    for (i64 b = 0; b < N; ++b)
+      for (i64 c = 0; c < N; ++c) {
+        const i64 i = IDX(N, a, b, c);
+        dxdt[i] = x[N3 + i];
+        dxdt[N3 + i] = ((-3.0) * H * x[N3 + i] - std::exp(2 * Psi[i]) * m * m * x[i]) +
+                       std::exp(4 * Psi[i]) * inv * stencil7(N, x, a, b, c);
+      }
+  });
+}
+
+// Sqrt_Potential_FRW_equation_kernel (equations_cuda.cu:83-106): V = m^2 f_a^2 (sqrt(1 + phi^2/f_a^2) - 1).
+extern "C" void orc_rhs_sqrt_potential(i64 N, const double *x, double *dxdt, double m, double f_a, double H,
+                                       double inv) {
+  const i64 N3 = N * N * N;
+  parallel_for(N, [&](i64 a) {
+    for (i64 b = 0; b < N; ++b)
+      for (i64 c = 0; c < N; ++c) {
+        const i64 i = IDX(N, a, b, c);
+        dxdt[i] = x[N3 + i];
+        dxdt[N3 + i] = ((-3.0) * H * x[N3 + i] - m * m * x[i] / std::sqrt(1.0 + x[i] * x[i] / (f_a * f_a))) +
+                       inv * stencil7(N, x, a, b, c);
+      }
+  });
+}
+
+// Sqrt_Potential_energy_density_kernel (equations_cuda.cu:422-444).
+extern "C" void orc_energy_density_sqrt(i64 N, const double *x, double *rho, double m, double f_a, double inv) {
+  const i64 N3 = N * N * N;
+  parallel_for(N, [&](i64 a) {
+    for (i64 b = 0; b < N; ++b)
+      for (i64 c = 0; c < N; ++c) {
+        const i64 i = IDX(N, a, b, c);
+        const double f = x[i];
+        const double dx = x[IDX(N, (a + 1) % N, b, c)] - x[IDX(N, (a + N - 1) % N, b, c)];
+        const double dy = x[IDX(N, a, (b + 1) % N, c)] - x[IDX(N, a, (b + N - 1) % N, c)];
+        const double dz = x[IDX(N, a, b, (c + 1) % N)] - x[IDX(N, a, b, (c + N - 1) % N)];
+        rho[i] = 0.5 * x[N3 + i] * x[N3 + i] + (0.5 / 4.0) * inv * (dx * dx + dy * dy + dz * dz) +
+                 m * m * f_a * f_a * (std::sqrt(1.0 + f * f / (f_a * f_a)) - 1.0);
+      }
+  });
+}
+
+// ComovingCurvatureEquationInFRW::operator() right-hand side given Psi, dPsi/dt
+// in real space (equations.cpp:357-380): CPU order, c-neighbours added in a
+// second pass.
+extern "C" void orc_rhs_variable_curvature(i64 N, const double *x, double *dxdt, const double *Psi,
+                                           const double *dPsidt, double m, double H, double inv) {
+  const i64 N3 = N * N * N;
+  parallel_for(N, [&](i64 a) {
+    for (i64 b = 0; b < N; ++b) {
+      const i64 r0 = IDX(N, a, b, 0);
+      const double *p0 = x + r0, *pi = x + N3 + r0, *ps = Psi + r0, *pd = dPsidt + r0;
+      const double *pa1 = x + IDX(N, (a + 1) % N, b, 0), *pam = x + IDX(N, (a + N - 1) % N, b, 0);
+      const double *pb1 = x + IDX(N, a, (b + 1) % N, 0), *pbm = x + IDX(N, a, (b + N - 1) % N, 0);
+      double *d0 = dxdt + r0, *d1 = dxdt + N3 + r0;
+      for (i64 c = 0; c < N; ++c) d0[c] = pi[c];
+      for (i64 c = 0; c < N; ++c)
+        d1[c] = ((-3.0 * H + 4.0 * pd[c]) * pi[c] - std::exp(2 * ps[c]) * m * m * p0[c]) +
+                std::exp(4 * ps[c]) * inv * ((((-6.0 * p0[c] + pa1[c]) + pam[c]) + pb1[c]) + pbm[c]);
+      for (i64 c = 1; c < N - 1; ++c) d1[c] = d1[c] + std::exp(4 * ps[c]) * inv * (p0[c + 1] + p0[c - 1]);
+      d1[0] = d1[0] + std::exp(4 * ps[0]) * inv * (p0[N - 1] + p0[1]);
+      d1[N - 1] = d1[N - 1] + std::exp(4 * ps[N - 1]) * inv * (p0[N - 2] + p0[0]);
+    }
+  });
+}
+
+// Approximate_Variable_Curvature_FRW_equation_kernel (equations_cuda.cu:299-329):
+// Psi, dPsi/dt on a coarse M^3 grid (here unpadded, IDX_OF(M,...)), exponentials
+// replaced by their second-order Taylor polynomials.
+extern "C" void orc_rhs_approx_curvature(i64 N, i64 M, const double *x, double *dxdt, const double *PsiM,
+                                         const double *dPsidtM, double m, double H, double inv) {
+  const i64 N3 = N * N * N;
+  const i64 sd = N / M, hs = sd / 2;
+  parallel_for(N, [&](i64 a) {
+    for (i64 b = 0; b < N; ++b)
+      for (i64 c = 0; c < N; ++c) {
+        const i64 i = IDX(N, a, b, c);
+        const i64 j = IDX(M, ((a + hs) % N) / sd, ((b + hs) % N) / sd, ((c + hs) % N) / sd);
+        const double P = PsiM[j];
+        dxdt[i] = x[N3 + i];
+        dxdt[N3 + i] = ((-3.0 * H + 4.0 * dPsidtM[j]) * x[N3 + i] - (1 + 2.0 * P + 2.0 * P * P) * m * m * x[i]) +
+                       (1 + 4.0 * P + 8.0 * P * P) * inv * stencil7(N, x, a, b, c);
+      }
+  });
+}
+
+// compute_cutoff_fouriers (fdm3d.cpp:105-130): half spectrum of N^3 -> of M^3.
+extern "C" void orc_cutoff_fouriers(i64 N, i64 M, const double *fft, double *out) {
+  const double scale_diff = static_cast<double>(N) / static_cast<double>(M);
+  for (i64 a = 0; a < M; ++a)
+    for (i64 b = 0; b < M; ++b)
+      for (i64 c = 0; c <= M / 2; ++c) {
+        const i64 aN = (a < M / 2) ? a : (N - (M - a));
+        const i64 bN = (b < M / 2) ? b : (N - (M - b));
+        const i64 iM = M * (M / 2 + 1) * a + (M / 2 + 1) * b + c;
+        const i64 iN = N * (N / 2 + 1) * aN + (N / 2 + 1) * bN + c;
+        out[2 * iM] = fft[2 * iN] / pow(scale_diff, 3);
+        out[2 * iM + 1] = fft[2 * iN + 1] / pow(scale_diff, 3);
+      }
+}
+
+// Curvature_energy_density_kernel (equations_cuda.cu:452-473), the fixed-curvature
+// energy density (CudaFixedCurvatureEquationInFRW::compute_energy_density, :660-675);
+// no CPU counterpart.
+extern "C" void orc_curvature_energy_density_cuda(i64 N, const double *x, const double *Psi, double *rho,
+                                                  double m, double inv) {
+  const i64 N3 = N * N * N;
+  parallel_for(N, [&](i64 a) {
+    for (i64 b = 0; b < N; ++b)
+      for (i64 c = 0; c < N; ++c) {
+        const i64 i = IDX(N, a, b, c);
+        const double f = x[i];
+        const double dx = x[IDX(N, (a + 1) % N, b, c)] - x[IDX(N, (a + N - 1) % N, b, c)];
+        const double dy = x[IDX(N, a, (b + 1) % N, c)] - x[IDX(N, a, (b + N - 1) % N, c)];
+        const double dz = x[IDX(N, a, b, (c + 1) % N)] - x[IDX(N, a, b, (c + N - 1) % N)];
+        rho[i] = std::exp(-2 * Psi[i]) * 0.5 * x[N3 + i] * x[N3 + i] +
+                 std::exp(2 * Psi[i]) * (0.5 / 4.0) * inv * (dx * dx + dy * dy + dz * dz) + 0.5 * m * m * f * f;
+      }
+  });
+}

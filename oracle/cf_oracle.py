@@ -87,6 +87,16 @@ def lib():
         L.orc_comoving_psi_fft.argtypes = [C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double]
         L.orc_curvature_energy_density.argtypes = [C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double]
         L.orc_ci_pade.argtypes = [C.c_double]
+        L.orc_rhs_fixed_curvature.argtypes = [C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double]
+        L.orc_rhs_sqrt_potential.argtypes = [C.c_int64, _dp, _dp, C.c_double, C.c_double, C.c_double,
+                                             C.c_double]
+        L.orc_energy_density_sqrt.argtypes = [C.c_int64, _dp, _dp, C.c_double, C.c_double, C.c_double]
+        L.orc_rhs_variable_curvature.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                                                 C.c_double]
+        L.orc_rhs_approx_curvature.argtypes = [C.c_int64, C.c_int64, _dp, _dp, _dp, _dp, C.c_double,
+                                               C.c_double, C.c_double]
+        L.orc_cutoff_fouriers.argtypes = [C.c_int64, C.c_int64, _dp, _dp]
+        L.orc_curvature_energy_density_cuda.argtypes = [C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double]
         L.orc_ci_pade.restype = C.c_double
         L.orc_set_threads(min(8, os.cpu_count() or 1))
         _LIB = L
@@ -345,6 +355,104 @@ def curvature_energy_density(N, x, Psi, m, inv):
     out = np.empty(N ** 3)
     lib().orc_curvature_energy_density(N, _p(x), _p(Psi), _p(out), m, inv)
     return out
+
+
+def curvature_energy_density_cuda(N, x, Psi, m, inv):
+    out = np.empty(N ** 3)
+    lib().orc_curvature_energy_density_cuda(N, _p(x), _p(Psi), _p(out), m, inv)
+    return out
+
+
+def cutoff_fouriers(N, M, fk):
+    out = np.empty(2 * M * M * (M // 2 + 1))
+    lib().orc_cutoff_fouriers(N, M, _p(fk), _p(out))
+    return out
+
+
+def comoving_factor(L, eta_t):
+    """(2 pi / L) eta(t) / sqrt(3) -- src/equations.cpp:196."""
+    return (2 * np.pi / L) * eta_t / np.sqrt(3.0)
+
+
+def comoving_psi(G, R_fft, factor, H_t):
+    """Psi, dPsi/dt in real space on a G^3 grid: transfer (equations.cpp:141-183)
+    then the unnormalised C2R of each (equations.cpp:232-233)."""
+    pk, dk = comoving_psi_fft(G, R_fft, factor, H_t)
+    return fft_c2r(G, pk), fft_c2r(G, dk)
+
+
+FIXED, SQRT, COMOVING, APPROX = 1, 2, 3, 4
+
+
+class VariantEquation:
+    """The curvature / potential variants with classic RK4 in the scale_sum order
+    (runge_kutta4_classic.hpp:87-138): y = x + dh*k, and
+    x <- (((x + dt6*k1) + dt3*k2) + dt3*k3) + dt6*k4, numpy ufuncs (no fused ops).
+
+    kind FIXED: field = Psi (N^3); SQRT: f_a; COMOVING: field = R_fft (half
+    spectrum of N^3); APPROX: field = R_fft and the coarse grid M."""
+
+    def __init__(self, kind, N, L, m, cosmo, field=None, f_a=1.0, M=0):
+        self.kind, self.N, self.L, self.m, self.cosmo = kind, N, L, m, cosmo
+        self.field, self.f_a, self.M = field, f_a, M
+        if kind == APPROX:
+            self.R_M = cutoff_fouriers(N, M, field)
+
+    def psi(self, t):
+        """Psi(t), dPsi/dt(t) on the grid the right-hand side reads."""
+        f = comoving_factor(self.L, cosmo_eta(self.cosmo, t))
+        H = cosmo_H(self.cosmo, t)
+        if self.kind == COMOVING:
+            return comoving_psi(self.N, self.field, f, H)
+        return comoving_psi(self.M, self.R_M, f, H)
+
+    def rhs(self, x, t):
+        N, L, m = self.N, self.L, self.m
+        H = cosmo_H(self.cosmo, t)
+        inv = inv_ah_sqr(N, L, cosmo_a(self.cosmo, t))
+        out = np.empty_like(x)
+        Lb = lib()
+        if self.kind == FIXED:
+            Lb.orc_rhs_fixed_curvature(N, _p(x), _p(out), _p(self.field), m, H, inv)
+        elif self.kind == SQRT:
+            Lb.orc_rhs_sqrt_potential(N, _p(x), _p(out), m, self.f_a, H, inv)
+        elif self.kind == COMOVING:
+            ps, dps = self.psi(t)
+            Lb.orc_rhs_variable_curvature(N, _p(x), _p(out), _p(ps), _p(dps), m, H, inv)
+        else:
+            ps, dps = self.psi(t)
+            Lb.orc_rhs_approx_curvature(N, self.M, _p(x), _p(out), _p(ps), _p(dps), m, H, inv)
+        return out
+
+    def rk4_step(self, x, t, dt):
+        dh, dt6, dt3 = dt / 2.0, dt / 6.0, dt / 3.0
+        th = t + dh
+        k1 = self.rhs(x, t)
+        acc = x + dt6 * k1
+        k2 = self.rhs(x + dh * k1, th)
+        acc = acc + dt3 * k2
+        k3 = self.rhs(x + dh * k2, th)
+        acc = acc + dt3 * k3
+        k4 = self.rhs(x + dt * k3, t + dt)
+        return acc + dt6 * k4
+
+    def rk4_run(self, x, t0, dt, n_steps, k0=0):
+        for k in range(n_steps):
+            x = self.rk4_step(x, t0 + float(k0 + k) * dt, dt)
+        return x
+
+    def energy_density(self, x, t):
+        N, m = self.N, self.m
+        inv = inv_ah_sqr(N, self.L, cosmo_a(self.cosmo, t))
+        if self.kind == SQRT:
+            out = np.empty(N ** 3)
+            lib().orc_energy_density_sqrt(N, _p(x), _p(out), m, self.f_a, inv)
+            return out
+        if self.kind == FIXED:
+            return curvature_energy_density_cuda(N, x, self.field, m, inv)
+        f = comoving_factor(self.L, cosmo_eta(self.cosmo, t))
+        ps = fft_c2r(N, comoving_psi_fft(N, self.field, f, cosmo_H(self.cosmo, t))[0])
+        return curvature_energy_density(N, x, ps, m, inv)
 
 
 if __name__ == "__main__":  # pragma: no cover

@@ -35,13 +35,26 @@ def run_kg(tmp_path, stepper, eq, N=32, L=40.0, m=1.0, lam=0.0, t0=1.0, t1=3.0, 
     print("case", " ".join(args[2:]), flush=True)
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    return out, dict(N=N, L=L, m=m, lam=lam, t0=t0, t1=t1, interval=interval, dt=dt, a1=a1, H1=H1, t1c=tt1)
+    return out, dict(N=N, L=L, m=m, lam=lam, t0=t0, t1=t1, interval=interval, dt=dt, a1=a1, H1=H1, t1c=tt1,
+                     eq=eq)
 
 
 def oracle_run(O, out, p, f32c=False):
     N = p["N"]
     x = np.fromfile(out + "state0.dat")
     cosmo = O.Cosmo.radiation(p["a1"], p["H1"], p["t1c"]) if p["H1"] != 0 else O.Cosmo.minkowski()
+    if p.get("eq") == "sqrt":
+        # the sqrt-potential variant (the lambda argument is f_a); the KG equation
+        # only supplies a(t) and the lattice constants to the observer
+        eq = O.Equation(N, p["L"], p["m"], 0.0, cosmo)
+        v = O.VariantEquation(O.SQRT, N, p["L"], p["m"], cosmo, f_a=p["lam"])
+        obs = O.ConstIntervalObserver(eq, p["t0"], p["t1"], p["interval"], rho_fn=v.energy_density)
+
+        def vstep(xx, t, dt):
+            xx[:] = v.rk4_step(xx, t, dt)
+
+        O.integrate_const(vstep, obs, x, p["t0"], p["t1"], p["dt"])
+        return x, obs
     eq = O.Equation(N, p["L"], p["m"], p["lam"], cosmo)
     obs = O.ConstIntervalObserver(eq, p["t0"], p["t1"], p["interval"])
     if f32c:
@@ -60,7 +73,8 @@ def oracle_run(O, out, p, f32c=False):
 
 @pytest.mark.parametrize("stepper,eq,lam", [("classic", "frw", 0.0), ("fused", "frw", 0.0),
                                             ("fused", "lambda", 0.1), ("fused", "kg", 0.0),
-                                            ("f32c", "frw", 0.0)])
+                                            ("f32c", "frw", 0.0), ("classic", "sqrt", 0.7),
+                                            ("fused", "sqrt", 0.7)])
 def test_kg_run_matches_oracle(O, tmp_path, stepper, eq, lam):
     out, p = run_kg(tmp_path, stepper, eq, lam=lam)
     x, obs = oracle_run(O, out, p, f32c=(stepper == "f32c"))
