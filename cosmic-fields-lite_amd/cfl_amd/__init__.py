@@ -95,6 +95,13 @@ def energy_density_f64(N, x, rho, m, lam, inv, stream=None):
     check(lib.cf_energy_density_f64(N, _ptr(x), _ptr(rho), m, lam, inv, _stream(stream)))
 
 
+def axis_average(N, rho, out=None, stream=None):
+    if out is None:
+        out = torch.empty(N * N, dtype=torch.float64, device=rho.device)
+    check(lib.cf_axis_average(N, _ptr(rho), _ptr(out), _stream(stream)))
+    return out
+
+
 # ---------------------------------------------------------------- fused RK4
 def work_bytes(N, precision=CF_PREC_F64):
     return int(lib.cf_rk4_work_bytes(N, precision))
@@ -224,6 +231,12 @@ class SpectrumPlan:
         if out is None:
             out = torch.empty(self.N ** 3, dtype=torch.float64, device="cuda")
         check(lib.cf_grf(self.h, _ptr(w), _ptr(sqrtP), _ptr(Psi), _ptr(out), _stream(stream)))
+        return out
+
+    def scaled_fourier_modes(self, f, kernel_by_s, out=None, stream=None):
+        if out is None:
+            out = torch.empty(self.N ** 3, dtype=torch.float64, device="cuda")
+        check(lib.cf_scaled_fourier_modes(self.h, _ptr(f), _ptr(kernel_by_s), _ptr(out), _stream(stream)))
         return out
 
     def init_grf_state(self, seed, kind, params, L, rms, state=None, stream=None):
@@ -397,3 +410,30 @@ def cutoff_fouriers(N, M, fk, stream=None):
     out = torch.empty(2 * M * M * (M // 2 + 1), dtype=torch.float64, device="cuda")
     check(lib.cf_cutoff_fouriers(N, M, _ptr(fk), _ptr(out), _stream(stream)))
     return out
+
+
+# ---------------------------------------------------------------- WKB extrapolation
+class WKB:
+    """WKB extrapolation of the state at t_i to later times (cf_wkb_*)."""
+
+    def __init__(self, p, state, t_i, stream=None):
+        h = C.c_void_p()
+        check(lib.cf_wkb_create(C.byref(p), _ptr(state), t_i, C.byref(h), _stream(stream)))
+        self.h, self.N = h, p.N
+
+    def close(self):
+        if self.h:
+            lib.cf_wkb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def evaluate(self, t, out=None, stream=None):
+        if out is None:
+            out = torch.empty(2 * self.N ** 3, dtype=torch.float64, device="cuda")
+        check(lib.cf_wkb_evaluate(self.h, t, _ptr(out), _stream(stream)))
+        return out

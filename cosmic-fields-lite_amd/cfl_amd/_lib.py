@@ -110,9 +110,14 @@ _SIGS = [
     ("cf_variant_psi", st, [vp, dbl, vp, vp, vp]),
     ("cf_comoving_psi_fft", st, [i64, vp, vp, vp, dbl, dbl, vp]),
     ("cf_cutoff_fouriers", st, [i64, i64, vp, vp, vp]),
+    ("cf_wkb_create", st, [C.POINTER(KGParams), vp, dbl, C.POINTER(vp), vp]),
+    ("cf_wkb_evaluate", st, [vp, dbl, vp, vp]),
+    ("cf_wkb_destroy", None, [vp]),
+    ("cf_wkb_tables_host", st, [i64, dbl, dbl, dbl, dbl, dbl, dbl, vp, vp, vp, vp]),
     ("cf_split_f64", st, [vp, vp, vp, i64, vp]),
     ("cf_join_f64", st, [vp, vp, vp, i64, vp]),
     ("cf_energy_density_f64", st, [i64, vp, vp, dbl, dbl, dbl, vp]),
+    ("cf_axis_average", st, [i64, vp, vp, vp]),
     ("cf_spectrum_plan_create", st, [i64, C.POINTER(vp)]),
     ("cf_spectrum_plan_destroy", st, [vp]),
     ("cf_spectrum_bins", i64, [i64]),
@@ -126,6 +131,7 @@ _SIGS = [
     ("cf_normals_host", st, [C.c_uint32, i64, vp]),
     ("cf_spectrum_table_host", st, [C.c_int, i64, dbl, vp, vp]),
     ("cf_grf", st, [vp, vp, vp, vp, vp, vp]),
+    ("cf_scaled_fourier_modes", st, [vp, vp, vp, vp, vp]),
     ("cf_rescale_rms", st, [vp, i64, dbl, vp]),
     ("cf_init_grf_state", st, [vp, C.c_uint32, C.c_int, vp, dbl, dbl, vp, vp]),
 ]

@@ -44,6 +44,21 @@ struct EnsStepCoef {
   double a[4], H[4];  // a(t) and H(t) at the four stage times
 };
 
+// A value known to be equal in all lanes (work-unit and lattice descriptors,
+// step scalars), read from the first lane so that it lives in scalar registers
+// and the index arithmetic on it is scalar.
+template <class V>
+__device__ __forceinline__ V wave_uniform(const V &v) {
+  static_assert(sizeof(V) % 4 == 0, "word-sized fields only");
+  struct Words {
+    uint32_t w[sizeof(V) / 4];
+  };
+  Words a = __builtin_bit_cast(Words, v);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(V) / 4); ++i) a.w[i] = __builtin_amdgcn_readfirstlane(a.w[i]);
+  return __builtin_bit_cast(V, a);
+}
+
 template <class T, bool F32C, bool LAM>
 __global__ __launch_bounds__(1024) void k_ens_tb(const EnsReal *__restrict__ reals,
                                                  const int32_t *__restrict__ unit_off,
@@ -54,12 +69,14 @@ __global__ __launch_bounds__(1024) void k_ens_tb(const EnsReal *__restrict__ rea
   const EnsStepCoef sc = steps[step];
   const int beg = unit_off[blockIdx.x], end = unit_off[blockIdx.x + 1];
   for (int i = beg; i < end; ++i) {
-    const EnsUnit e = units[i];
-    const EnsReal R = reals[e.real];
+    const EnsUnit e = wave_uniform(units[i]);
+    const EnsReal R = wave_uniform(reals[e.real]);
     TbStep k;
+    // The step scalars are the same for the whole block; moving them to scalar
+    // registers keeps the vector registers for the step body.
     for (int s = 0; s < 4; ++s) {
-      k.c_h[s] = (-3.0) * sc.H[s];
-      k.inv[s] = 1.0 / ((R.L / R.N) * (R.L / R.N)) / (sc.a[s] * sc.a[s]);  // cf_inv_ah_sqr
+      k.c_h[s] = wave_uniform((-3.0) * sc.H[s]);
+      k.inv[s] = wave_uniform(1.0 / ((R.L / R.N) * (R.L / R.N)) / (sc.a[s] * sc.a[s]));  // cf_inv_ah_sqr
     }
     k.c_m = c_m;
     k.lam = lam;
@@ -74,7 +91,7 @@ __global__ __launch_bounds__(1024) void k_ens_tb(const EnsReal *__restrict__ rea
     u.a0 = e.a0;
     u.a1 = e.a1;
     u.T = e.T;
-    tb_unit<T, F32C, LAM>(lds, u, k, (const T *)R.x[parity], R.lo[parity], (T *)R.x[1 - parity], R.lo[1 - parity]);
+    tb_unit<T, F32C, LAM, false>(lds, u, k, (const T *)R.x[parity], R.lo[parity], (T *)R.x[1 - parity], R.lo[1 - parity]);
     __syncthreads();  // the next unit rewrites the plane buffers
   }
 }

@@ -126,6 +126,26 @@ __global__ void k_grf_scale(double2 *__restrict__ fk, const double *__restrict__
   }
 }
 
+// Multiply the half spectrum by kernel(s)/N^3 (compute_field_with_scaled_fourier_modes,
+// fdm3d.cpp:84-97; the kernel is tabulated over s = |n|^2).
+__global__ void k_scale_modes(double2 *__restrict__ fk, const double *__restrict__ kern, int64_t N, int64_t H) {
+  const int64_t total = N * N * H;
+  const double n3 = (double)(N * N * N);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = e % H;
+    const int64_t r = e / H;
+    const int64_t b = r % N, a = r / N;
+    const int64_t as = (a <= N / 2) ? a : N - a;
+    const int64_t bs = (b <= N / 2) ? b : N - b;
+    const double scale = kern[as * as + bs * bs + c * c] / n3;
+    double2 v = fk[e];
+    v.x *= scale;
+    v.y *= scale;
+    fk[e] = v;
+  }
+}
+
 __global__ void k_mul_exp(double *__restrict__ w, const double *__restrict__ psi, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -428,6 +448,20 @@ cf_status cf_grf(cf_spectrum_plan *p, double *w, const double *sqrtP, const doub
   CF_TRY(d2z_exec(p, w, p->cbuf0, st));
   hipLaunchKernelGGL(k_grf_scale, dim3(grid_cap(p->nhalf, 256)), dim3(256), 0, st, (double2 *)p->cbuf0,
                      sqrtP, p->N, p->H);
+  CF_LAUNCHED();
+  return z2d_exec(p, p->cbuf0, out, st);
+}
+
+cf_status cf_scaled_fourier_modes(cf_spectrum_plan *p, const double *f, const double *kernel_by_s, double *out,
+                                  cf_stream stream) {
+  CF_TRY(plan_check(p));
+  CF_REQUIRE(f && kernel_by_s && out, "cf_scaled_fourier_modes: null buffer");
+  hipStream_t st = S(stream);
+  const size_t n3 = (size_t)p->N * p->N * p->N;
+  CF_HIP(hipMemcpyAsync(p->rbuf, f, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  CF_TRY(d2z_exec(p, p->rbuf, p->cbuf0, st));
+  hipLaunchKernelGGL(k_scale_modes, dim3(grid_cap(p->nhalf, 256)), dim3(256), 0, st, (double2 *)p->cbuf0,
+                     kernel_by_s, p->N, p->H);
   CF_LAUNCHED();
   return z2d_exec(p, p->cbuf0, out, st);
 }

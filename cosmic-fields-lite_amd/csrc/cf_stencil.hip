@@ -159,6 +159,17 @@ __global__ __launch_bounds__(256) void k_energy_density(Tiling tl, RhoCoef rc, c
   }
 }
 
+// rho.reshaped(N*N, N).rowwise().mean() (observer.hpp:114): for each (b, c) the
+// sum over a in increasing a, then one division by N.
+__global__ void k_axis_average(int64_t N, const double *__restrict__ rho, double *__restrict__ avg) {
+  const int64_t n2 = N * N;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x) {
+    double s = rho[i];
+    for (int64_t a = 1; a < N; ++a) s = s + rho[a * n2 + i];
+    avg[i] = s / (double)N;
+  }
+}
+
 __global__ void k_split(const double *__restrict__ x, float *__restrict__ hi, float *__restrict__ lo,
                         int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -331,6 +342,16 @@ cf_status cf_join_f64(const float *hi, const float *lo, double *x, int64_t n, cf
   int64_t g = cdiv(n, 256);
   if (g > 65536) g = 65536;
   hipLaunchKernelGGL(k_join, dim3((unsigned)g), dim3(256), 0, S(stream), hi, lo, x, n);
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
+cf_status cf_axis_average(int64_t N, const double *rho, double *avg, cf_stream stream) {
+  CF_REQUIRE(valid_N(N), "cf_axis_average: unsupported N=%lld", (long long)N);
+  CF_REQUIRE(rho && avg && (const void *)rho != (const void *)avg, "cf_axis_average: null or aliased buffers");
+  int64_t g = cdiv(N * N, 256);
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_axis_average, dim3((unsigned)g), dim3(256), 0, S(stream), N, rho, avg);
   CF_LAUNCHED();
   return CF_OK;
 }

@@ -30,6 +30,8 @@
 // therefore bit-identical to the four-kernel path.
 #pragma once
 
+#include <type_traits>
+
 #include "cf_common.hpp"
 
 namespace cf {
@@ -67,6 +69,34 @@ __device__ __forceinline__ double rnd(double v) {
   else return v;
 }
 
+// Buffer-resource access to one lattice plane (gfx9 raw buffer, stride 0,
+// num_records = plane bytes; word 3 = 0x00020000 as for all gfx9 parts). An
+// offset of TB_OOB is out of range: stores to it are dropped, loads return 0.
+constexpr uint32_t TB_OOB = 0x80000000u;
+typedef unsigned int tb_u2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tb_plane(const void *base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, bytes, 0x00020000);
+}
+__device__ __forceinline__ void tb_store(double v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tb_u2, v), r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void tb_store(float v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)off, 0, 0);
+}
+__device__ __forceinline__ float tb_load_f32(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+
+// The new state of one iteration, stored two iterations later.
+template <class T>
+struct TbPending {
+  T f, p;         // phi, pi (FP64) or their hi parts (F32C)
+  float lf, lp;   // F32C: lo parts
+  uint32_t off;   // lane offset in the plane, or TB_OOB
+  int64_t plane;  // plane offset (times N^2)
+};
+
 // Two sets of four plane buffers: iteration j writes the four stage inputs
 // (all known when it starts: they are the previous iteration's outputs) into set
 // j&1, then one barrier. Set j&1 was last read in iteration j-2, which every
@@ -75,13 +105,15 @@ __device__ __forceinline__ double rnd(double v) {
 template <class T>
 using TbLds = T[2][4][TBR][TBR];
 
-template <class T, bool F32C, bool LAM>
+template <class T, bool F32C, bool LAM, bool DEFER = true>
 __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const TbStep &k, const T *__restrict__ xin,
                                         const float *__restrict__ lin, T *__restrict__ xout,
                                         float *__restrict__ lout) {
   const int tid = threadIdx.x;
   const int r = tid >> 5, cc = tid & 31;
+  static_assert(!F32C || sizeof(T) == sizeof(float), "the compensated mode stores float");
   const int64_t N = u.N, NN = N * N, N3 = NN * N;
+  const int plane_bytes = (int)(NN * (int64_t)sizeof(T)), plane_bytes_f = (int)(NN * (int64_t)sizeof(float));
   int64_t b = (u.b0 - TBH + r) % N;
   if (b < 0) b += N;
   int64_t c = (u.c0 - TBH + cc) % N;
@@ -103,39 +135,79 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     return off == N3 ? 0 : off;
   };
 
-  double xf0 = 0, xf1 = 0, xf2 = 0, xf3 = 0, xf4 = 0;  // x phi at P, P-1, ..., P-4
-  double xp0 = 0, xp1 = 0, xp2 = 0, xp3 = 0, xp4 = 0;  // x pi
-  double y1f1 = 0, y1f2 = 0, y1p1 = 0;                  // y1 phi at P-2, P-3 (start of iteration); pi at P-2
-  double y2f1 = 0, y2f2 = 0, y2p1 = 0;                  // y2 phi at P-3, P-4; pi at P-3
-  double y3f1 = 0, y3f2 = 0, y3p1 = 0;                  // y3 phi at P-4, P-5; pi at P-4
-  double af1 = 0, af2 = 0, af3 = 0, ap1 = 0, ap2 = 0, ap3 = 0;  // acc at P-2, P-3, P-4
+  // In the compensated mode every value carried from plane to plane is exactly
+  // representable in float (the state hi, the rounded stage inputs and the rounded
+  // increment), so the histories are held as float (half the registers) and
+  // widened where they are used; the arithmetic stays double.
+  typedef std::conditional_t<F32C, float, double> Hv;
+  Hv xf0 = 0, xf1 = 0, xf2 = 0, xf3 = 0, xf4 = 0;  // x phi at P, P-1, ..., P-4
+  Hv xp0 = 0, xp1 = 0, xp2 = 0, xp3 = 0, xp4 = 0;  // x pi
+  Hv y1f1 = 0, y1f2 = 0, y1p1 = 0;                  // y1 phi at P-2, P-3 (start of iteration); pi at P-2
+  Hv y2f1 = 0, y2f2 = 0, y2p1 = 0;                  // y2 phi at P-3, P-4; pi at P-3
+  Hv y3f1 = 0, y3f2 = 0, y3p1 = 0;                  // y3 phi at P-4, P-5; pi at P-4
+  Hv af1 = 0, af2 = 0, af3 = 0, ap1 = 0, ap2 = 0, ap3 = 0;  // acc (FP64) / increment D (F32C) at P-2, P-3, P-4
 
   const int iters = (int)(u.a1 - u.a0) + 8;
-  // two planes of prefetch: pf0 = x at P, pf1 = x at P+1
+  // Memory pipeline. Two prefetch slots are used alternately: iteration j takes
+  // x at plane P from slot j&1 (loaded two iterations earlier) and refills it
+  // with plane P+2. The new state of iteration j is stored at the start of
+  // iteration j+2. The loop is unrolled by two, so slots and pending stores are
+  // fixed registers and every wait is for an operation issued at least an
+  // iteration earlier (DEFER = false stores at the end of the iteration, for
+  // callers with less register room). Stores go through buffer resources over one plane: lanes
+  // that do not store get an offset past its end, which the hardware drops
+  // (loads of it return 0), so no branch surrounds a memory operation.
   int64_t off_ld = wrap_plane(u.a0 - TBH);
-  double pf0f = (double)xin[off_ld + col], pf0p = (double)xin[N3 + off_ld + col];
+  Hv s0f = (Hv)xin[off_ld + col], s0p = (Hv)xin[N3 + off_ld + col];
   off_ld = next_plane(off_ld);
-  double pf1f = (double)xin[off_ld + col], pf1p = (double)xin[N3 + off_ld + col];
+  Hv s1f = (Hv)xin[off_ld + col], s1p = (Hv)xin[N3 + off_ld + col];
   int64_t off_st = wrap_plane(u.a0 - TBH - 4);  // plane P-4
+  TbPending<T> q0{}, q1{};
+  q0.off = q1.off = TB_OOB;
+  // F32C: the compensation term of output plane P-4 is prefetched the same way
+  // (slot j&1, loaded two iterations ahead; lanes that will not store read 0).
+  float l0f = 0.0f, l0p = 0.0f, l1f = 0.0f, l1p = 0.0f;
+  int64_t off_lo = off_st;
+  auto lo_off = [&](int jj) -> uint32_t {
+    return ((jj >= 8) && out_ok) ? (uint32_t)(col * (int64_t)sizeof(float)) : TB_OOB;
+  };
+  if constexpr (F32C) {
+    l0f = tb_load_f32(tb_plane(lin + off_lo, plane_bytes_f), lo_off(0));
+    l0p = tb_load_f32(tb_plane(lin + N3 + off_lo, plane_bytes_f), lo_off(0));
+    off_lo = next_plane(off_lo);
+    l1f = tb_load_f32(tb_plane(lin + off_lo, plane_bytes_f), lo_off(1));
+    l1p = tb_load_f32(tb_plane(lin + N3 + off_lo, plane_bytes_f), lo_off(1));
+  }
 
-  for (int j = 0; j < iters; ++j) {
-    xf0 = pf0f;
-    xp0 = pf0p;
-    pf0f = pf1f;
-    pf0p = pf1p;
-    off_ld = next_plane(off_ld);  // plane P+2
-    pf1f = (double)xin[off_ld + col];
-    pf1p = (double)xin[N3 + off_ld + col];
-    const bool store = (j >= 8) && out_ok;
+  auto flush = [&](const TbPending<T> &q) {
+    tb_store(q.f, tb_plane(xout + q.plane, plane_bytes), q.off);
+    tb_store(q.p, tb_plane(xout + N3 + q.plane, plane_bytes), q.off);
+    if constexpr (F32C) {
+      tb_store(q.lf, tb_plane(lout + q.plane, plane_bytes_f), q.off);
+      tb_store(q.lp, tb_plane(lout + N3 + q.plane, plane_bytes_f), q.off);
+    }
+  };
+
+  auto iteration = [&](const int j, Hv &slf, Hv &slp, float &llf, float &llp, T(*buf)[TBR][TBR],
+                       TbPending<T> &q) {
+    xf0 = slf;
+    xp0 = slp;
     double lo_f = 0.0, lo_p = 0.0;
     if constexpr (F32C) {
-      if (store) {
-        lo_f = (double)lin[off_st + col];
-        lo_p = (double)lin[N3 + off_st + col];
-      }
+      lo_f = (double)llf;
+      lo_p = (double)llp;
     }
+    if constexpr (DEFER) flush(q);  // the new state of iteration j-2
+    const uint32_t st_off = ((j >= 8) && out_ok) ? (uint32_t)(col * (int64_t)sizeof(T)) : TB_OOB;
+    if constexpr (F32C) {
+      off_lo = next_plane(off_lo);  // plane P-2, the output plane of iteration j+2
+      llf = tb_load_f32(tb_plane(lin + off_lo, plane_bytes_f), lo_off(j + 2));
+      llp = tb_load_f32(tb_plane(lin + N3 + off_lo, plane_bytes_f), lo_off(j + 2));
+    }
+    off_ld = next_plane(off_ld);  // plane P+2
+    slf = (Hv)xin[off_ld + col];
+    slp = (Hv)xin[N3 + off_ld + col];
 
-    T(*buf)[TBR][TBR] = lds[j & 1];
     buf[0][r][cc] = (T)xf1;
     buf[1][r][cc] = (T)y1f1;
     buf[2][r][cc] = (T)y2f1;
@@ -143,56 +215,61 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     __syncthreads();
 
     // ---- stage 1 at P-1: k1 = f(x)
-    double kf = xp1;
-    double kp = tb_rhs(xf1, xf0, xf2, (double)buf[0][rp][cc], (double)buf[0][rm][cc], (double)buf[0][r][cp],
-                       (double)buf[0][r][cm], xp1, k.c_h[0], k.c_m, k.lam, k.inv[0], LAM);
-    const double y1f0 = rnd<F32C>(xf1 + k.dh * kf);
-    const double y1p0 = rnd<F32C>(xp1 + k.dh * kp);
-    double af0, ap0;
+    double kf = (double)xp1;
+    double kp = tb_rhs((double)xf1, (double)xf0, (double)xf2, (double)buf[0][rp][cc], (double)buf[0][rm][cc],
+                       (double)buf[0][r][cp], (double)buf[0][r][cm], (double)xp1, k.c_h[0], k.c_m, k.lam,
+                       k.inv[0], LAM);
+    const Hv y1f0 = (Hv)rnd<F32C>((double)xf1 + k.dh * kf);
+    const Hv y1p0 = (Hv)rnd<F32C>((double)xp1 + k.dh * kp);
+    Hv af0, ap0;
     if constexpr (F32C) {
-      af0 = rnd<true>(k.dt6 * kf);
-      ap0 = rnd<true>(k.dt6 * kp);
+      af0 = (Hv)(k.dt6 * kf);
+      ap0 = (Hv)(k.dt6 * kp);
     } else {
       af0 = xf1 + k.dt6 * kf;
       ap0 = xp1 + k.dt6 * kp;
     }
 
     // ---- stage 2 at P-2: k2 = f(y1)
-    kf = y1p1;
-    kp = tb_rhs(y1f1, y1f0, y1f2, (double)buf[1][rp][cc], (double)buf[1][rm][cc], (double)buf[1][r][cp],
-                (double)buf[1][r][cm], y1p1, k.c_h[1], k.c_m, k.lam, k.inv[1], LAM);
-    const double y2f0 = rnd<F32C>(xf2 + k.dh * kf);
-    const double y2p0 = rnd<F32C>(xp2 + k.dh * kp);
-    af1 = rnd<F32C>(af1 + k.dt3 * kf);
-    ap1 = rnd<F32C>(ap1 + k.dt3 * kp);
+    kf = (double)y1p1;
+    kp = tb_rhs((double)y1f1, (double)y1f0, (double)y1f2, (double)buf[1][rp][cc], (double)buf[1][rm][cc],
+                (double)buf[1][r][cp], (double)buf[1][r][cm], (double)y1p1, k.c_h[1], k.c_m, k.lam, k.inv[1],
+                LAM);
+    const Hv y2f0 = (Hv)rnd<F32C>((double)xf2 + k.dh * kf);
+    const Hv y2p0 = (Hv)rnd<F32C>((double)xp2 + k.dh * kp);
+    af1 = (Hv)((double)af1 + k.dt3 * kf);
+    ap1 = (Hv)((double)ap1 + k.dt3 * kp);
 
     // ---- stage 3 at P-3: k3 = f(y2)
-    kf = y2p1;
-    kp = tb_rhs(y2f1, y2f0, y2f2, (double)buf[2][rp][cc], (double)buf[2][rm][cc], (double)buf[2][r][cp],
-                (double)buf[2][r][cm], y2p1, k.c_h[2], k.c_m, k.lam, k.inv[2], LAM);
-    const double y3f0 = rnd<F32C>(xf3 + k.dt * kf);
-    const double y3p0 = rnd<F32C>(xp3 + k.dt * kp);
-    af2 = rnd<F32C>(af2 + k.dt3 * kf);
-    ap2 = rnd<F32C>(ap2 + k.dt3 * kp);
+    kf = (double)y2p1;
+    kp = tb_rhs((double)y2f1, (double)y2f0, (double)y2f2, (double)buf[2][rp][cc], (double)buf[2][rm][cc],
+                (double)buf[2][r][cp], (double)buf[2][r][cm], (double)y2p1, k.c_h[2], k.c_m, k.lam, k.inv[2],
+                LAM);
+    const Hv y3f0 = (Hv)rnd<F32C>((double)xf3 + k.dt * kf);
+    const Hv y3p0 = (Hv)rnd<F32C>((double)xp3 + k.dt * kp);
+    af2 = (Hv)((double)af2 + k.dt3 * kf);
+    ap2 = (Hv)((double)ap2 + k.dt3 * kp);
 
     // ---- stage 4 at P-4: k4 = f(y3); new state
-    kf = y3p1;
-    kp = tb_rhs(y3f1, y3f0, y3f2, (double)buf[3][rp][cc], (double)buf[3][rm][cc], (double)buf[3][r][cp],
-                (double)buf[3][r][cm], y3p1, k.c_h[3], k.c_m, k.lam, k.inv[3], LAM);
-    if (store) {
-      if constexpr (F32C) {
-        const double s0 = (xf4 + lo_f) + (af3 + k.dt6 * kf);
-        const double s1 = (xp4 + lo_p) + (ap3 + k.dt6 * kp);
-        const float h0 = (float)s0, h1 = (float)s1;
-        xout[off_st + col] = h0;
-        xout[N3 + off_st + col] = h1;
-        lout[off_st + col] = (float)(s0 - (double)h0);
-        lout[N3 + off_st + col] = (float)(s1 - (double)h1);
-      } else {
-        xout[off_st + col] = af3 + k.dt6 * kf;
-        xout[N3 + off_st + col] = ap3 + k.dt6 * kp;
-      }
+    kf = (double)y3p1;
+    kp = tb_rhs((double)y3f1, (double)y3f0, (double)y3f2, (double)buf[3][rp][cc], (double)buf[3][rm][cc],
+                (double)buf[3][r][cp], (double)buf[3][r][cm], (double)y3p1, k.c_h[3], k.c_m, k.lam, k.inv[3],
+                LAM);
+    if constexpr (F32C) {
+      const double s0 = ((double)xf4 + lo_f) + ((double)af3 + k.dt6 * kf);
+      const double s1 = ((double)xp4 + lo_p) + ((double)ap3 + k.dt6 * kp);
+      const float h0 = (float)s0, h1 = (float)s1;
+      q.f = h0;
+      q.p = h1;
+      q.lf = (float)(s0 - (double)h0);
+      q.lp = (float)(s1 - (double)h1);
+    } else {
+      q.f = af3 + k.dt6 * kf;
+      q.p = ap3 + k.dt6 * kp;
     }
+    q.off = st_off;
+    q.plane = off_st;
+    if constexpr (!DEFER) flush(q);
     off_st = next_plane(off_st);
 
     // ---- advance the histories by one plane
@@ -203,6 +280,16 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     y3f2 = y3f1; y3f1 = y3f0; y3p1 = y3p0;
     af3 = af2; af2 = af1; af1 = af0;
     ap3 = ap2; ap2 = ap1; ap1 = ap0;
+  };
+  int j = 0;
+  for (; j + 1 < iters; j += 2) {
+    iteration(j, s0f, s0p, l0f, l0p, lds[0], q0);
+    iteration(j + 1, s1f, s1p, l1f, l1p, lds[1], q1);
+  }
+  if (j < iters) iteration(j, s0f, s0p, l0f, l0p, lds[0], q0);
+  if constexpr (DEFER) {
+    flush(q0);
+    flush(q1);
   }
 }
 

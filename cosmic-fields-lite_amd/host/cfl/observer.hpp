@@ -90,13 +90,14 @@ struct ConstIntervalObserver {
         write_vector_to_filename_template(to_host(s), dir + "rho_spectrum_%d.dat", idx);
       }
       if constexpr (save_density) {
-        // slice at a = 0 and the average over a (observer.hpp:110-119)
-        std::vector<double> h = to_host(rho);
+        // slice at a = 0 and the average over a (observer.hpp:110-119), both
+        // taken on the device: 2 N^2 values cross PCIe instead of N^3
         const std::size_t n2 = (std::size_t)(N * N);
-        std::vector<double> slice(h.begin(), h.begin() + n2), avg(n2, 0.0);
-        for (long long a = 0; a < N; ++a)
-          for (std::size_t i = 0; i < n2; ++i) avg[i] += h[a * n2 + i];
-        for (auto &v : avg) v /= (double)N;
+        std::vector<double> slice(n2), avg(n2);
+        hip_vector<double> avg_d(n2);
+        check(cf_axis_average(N, rho.data(), avg_d.data(), default_stream()), "cf_axis_average");
+        check(cf_d2h(slice.data(), rho.data(), n2 * sizeof(double), default_stream()), "rho slice");
+        check(cf_d2h(avg.data(), avg_d.data(), n2 * sizeof(double), default_stream()), "rho axis average");
         write_vector_to_filename_template(slice, dir + "rho_slice_%d.dat", idx);
         write_vector_to_filename_template(avg, dir + "rho_axis_average_%d.dat", idx);
       }

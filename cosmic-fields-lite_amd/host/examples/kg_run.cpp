@@ -8,7 +8,8 @@
 //             fused   (cfl::hip_runge_kutta4: fused stage kernels)
 //             f32c    (cfl::hip_runge_kutta4_f32c: FP32-compensated storage)
 //   equation: kg | frw | lambda | sqrt (sqrt potential; the lambda argument is f_a)
-// Writes state0.dat (initial state), the observer's spectra, t_list.dat and state.dat.
+// Writes state0.dat (initial state), the observer's spectra and density slices, t_list.dat
+// and state.dat.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -37,7 +38,7 @@ static int run(const std::string &dir, const std::string &stepper, const RunPara
   Workspace workspace(param, init);
   cfl::write_vector_to_file(cfl::to_host(workspace.state), dir + "state0.dat");
   Equation eqn(workspace);
-  cfl::ConstIntervalObserver<Equation, true, true, false> observer(dir, param, eqn);
+  cfl::ConstIntervalObserver<Equation, true, true, true> observer(dir, param, eqn);
   std::size_t steps = 0;
   if (stepper == "classic") {
     steps = cfl::odeint_lite::integrate_const(cfl::odeint_lite::runge_kutta4_classic<>(), eqn, workspace.state,

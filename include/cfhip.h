@@ -199,6 +199,10 @@ cf_status cf_adjoint_spectrum_loss(cf_spectrum_plan *plan, const cf_kg_params *p
  * equations_cuda.cu:332-354 via :537-553 and :591-607): rho (N^3) from the state. */
 cf_status cf_energy_density_f64(int64_t N, const double *x_dev, double *rho_dev, double m,
                                 double lambda, double inv_ah2, cf_stream stream);
+/* The observer's density average over a (src/observer.hpp:110-119,
+ * rho.reshaped(N*N, N).rowwise().mean()): avg_dev[b*N + c] = (sum over a, in order) / N.
+ * The a = 0 slice is simply the first N^2 entries of rho. */
+cf_status cf_axis_average(int64_t N, const double *rho_dev, double *avg_dev, cf_stream stream);
 
 /* ---------------------------------------------------------------- equation variants
  * The curvature / potential equations, FP64:
@@ -246,6 +250,22 @@ cf_status cf_comoving_psi_fft(int64_t G, const double *R_fft_dev, double *psi_ff
 /* compute_cutoff_fouriers (fdm3d.cpp:105-130; fdm3d_cuda.cu:117-141): half spectrum of N^3
  * -> half spectrum of M^3, divided by (N/M)^3. */
 cf_status cf_cutoff_fouriers(int64_t N, int64_t M, const double *in_dev, double *out_dev, cf_stream stream);
+
+/* ---------------------------------------------------------------- WKB extrapolation
+ * WKBSolutionForKleinGordonEquationInFRW (src/wkb.cpp:5-121): each Fourier mode of the
+ * state at t_i advanced to a later t by its closed-form WKB phase, for a
+ * radiation-era cosmology (p = 1, t1 = 1/(2 H1), as wkb.cpp:7-8 asserts) and m > 0.
+ * The handle keeps the scaled half spectra of the state; evaluate writes the
+ * extrapolated state (2N^3) for any t. N <= 1024. */
+typedef struct cf_wkb_s *cf_wkb;
+cf_status cf_wkb_create(const cf_kg_params *p, const double *state_dev, double t_i, cf_wkb *out, cf_stream stream);
+cf_status cf_wkb_evaluate(cf_wkb w, double t, double *state_out_dev, cf_stream stream);
+void cf_wkb_destroy(cf_wkb w);
+/* The per-s tables the evaluation uses (compute_phase_integral and omega_eff,
+ * wkb.cpp:40-96): cos(phase), sin(phase), sqrt(omega_eff(t_i)); phase_host may be NULL.
+ * Each table holds cf_spectrum_bins(N) doubles. */
+cf_status cf_wkb_tables_host(int64_t N, double L, double m, double a1, double H1, double t_i, double t_f,
+                             double *cos_host, double *sin_host, double *sqrt_omega_host, double *phase_host);
 
 /* ---------------------------------------------------------------- FFT and spectra
  * A spectrum plan owns the hipFFT D2Z/Z2D plans of an N^3 lattice with one shared
@@ -296,6 +316,10 @@ cf_status cf_spectrum_table_host(int kind, int64_t N, double L, const double *pa
  * w_dev is overwritten. */
 cf_status cf_grf(cf_spectrum_plan *plan, double *w_dev, const double *sqrtP_by_s_dev,
                  const double *Psi_dev, double *out_dev, cf_stream stream);
+/* compute_field_with_scaled_fourier_modes (src/fdm3d.cpp:76-102): out = C2R(R2C(f) * kernel(s) / N^3),
+ * kernel tabulated over s = |n|^2 (cf_spectrum_bins(N) entries, k = sqrt(s) 2 pi / L). f is preserved. */
+cf_status cf_scaled_fourier_modes(cf_spectrum_plan *plan, const double *f_dev, const double *kernel_by_s_dev,
+                                  double *out_dev, cf_stream stream);
 /* x *= target / rms(x), rms from a fixed-order device reduction (bit-reproducible). */
 cf_status cf_rescale_rms(double *x_dev, int64_t n, double target, cf_stream stream);
 /* Workload initial condition: phi = GRF(seed, spectrum kind/params) rescaled to rms

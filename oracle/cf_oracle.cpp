@@ -832,3 +832,72 @@ extern "C" void orc_curvature_energy_density_cuda(i64 N, const double *x, const 
       }
   });
 }
+
+// ---------------------------------------------------------------------------
+// WKB extrapolation: WKBSolutionForKleinGordonEquationInFRW (wkb.cpp:5-121),
+// radiation era (p = 1, t1 = 1/(2 H1)).
+// ---------------------------------------------------------------------------
+// Per s = |n|^2: phase(s) = integral of omega_eff from t_i to t_f in closed form
+// (wkb.cpp:40-79), omega_eff_i = sqrt(m^2 + k^2/(2 a1^2 H1 t_i) + 3/(16 t_i^2))
+// (wkb.cpp:96); out: cos(phase), sin(phase), sqrt(omega_eff_i), phase.
+extern "C" void orc_wkb_tables(i64 N, double L, double m, double a1, double H1, double t_i, double t_f,
+                               double *cos_t, double *sin_t, double *sqw_t, double *phase_t) {
+  const i64 nb = 3 * (N / 2) * (N / 2) + 1;
+  const double pi = 3.141592653589793;
+  for (i64 s = 0; s < nb; ++s) {
+    const double k_sqr = s * std::pow(2 * pi / L, 2);
+    auto Q = [&](double t) {
+      return std::sqrt(H1 * (8 * k_sqr * t + std::pow(a1, 2) * H1 * (3 + 16 * std::pow(m, 2) * std::pow(t, 2))));
+    };
+    auto B = [&](double t) { return (3 * std::pow(a1, 2) * H1 + 4 * k_sqr * t) / (std::sqrt(3) * a1 * Q(t)); };
+    auto D = [&](double t) { return k_sqr + a1 * m * (4 * a1 * H1 * m * t + Q(t)); };
+    const double phase = Q(t_f) / (4. * a1 * H1) - Q(t_i) / (4. * a1 * H1) +
+                         (std::sqrt(3) * std::log((1 - B(t_f)) / (1 - B(t_i)))) / 8. -
+                         (std::sqrt(3) * std::log((1 + B(t_f)) / (1 + B(t_i)))) / 8. +
+                         (k_sqr * std::log(D(t_f) / D(t_i))) / (4. * std::pow(a1, 2) * H1 * m);
+    const double omega_eff_i = std::sqrt(m * m + k_sqr / (2 * a1 * a1 * H1 * t_i) + 3 / (16 * t_i * t_i));
+    cos_t[s] = cos(phase);
+    sin_t[s] = sin(phase);
+    sqw_t[s] = std::sqrt(omega_eff_i);
+    if (phase_t) phase_t[s] = phase;
+  }
+}
+
+// Constructor (wkb.cpp:5-20): the two half spectra of the state, scaled by a^1.5,
+// the second combined with 1.5 H times the (scaled) first.
+extern "C" void orc_wkb_init(i64 N, const double *state, double a_t, double H_t, double *ffts) {
+  const i64 fs = 2 * N * N * (N / 2 + 1);
+  orc_fft_r2c(N, state, ffts);
+  orc_fft_r2c(N, state + N * N * N, ffts + fs);
+  const double a15 = pow(a_t, 1.5);
+  for (i64 i = 0; i < fs; ++i) ffts[i] *= a15;
+  for (i64 i = 0; i < fs; ++i) ffts[fs + i] = a15 * ffts[fs + i] + 1.5 * H_t * ffts[i];
+}
+
+// evaluate_at (wkb.cpp:24-121) given the tables of orc_wkb_tables and a(t), H(t).
+extern "C" void orc_wkb_evaluate(i64 N, const double *ffts, const double *cos_t, const double *sin_t,
+                                 const double *sqw_t, double a_t, double H_t, double *out) {
+  const i64 H = N / 2 + 1, fs = 2 * N * N * H, N3 = N * N * N;
+  std::vector<double> np((size_t)fs), nd((size_t)fs);
+  for (i64 a = 0; a < N; ++a)
+    for (i64 b = 0; b < N; ++b)
+      for (i64 c = 0; c <= N / 2; ++c) {
+        const i64 as = (a <= N / 2) ? a : (N - a), bs = (b <= N / 2) ? b : (N - b);
+        const i64 s = as * as + bs * bs + c * c;
+        const i64 o = 2 * (N * H * a + H * b + c);
+        const double cv = cos_t[s], sv = sin_t[s], sw = sqw_t[s];
+        for (int q = 0; q < 2; ++q) {
+          np[o + q] = ffts[o + q] * cv + ffts[fs + o + q] * sv / sw;
+          nd[o + q] = ffts[fs + o + q] * cv - ffts[o + q] * sv * sw;
+        }
+      }
+  std::vector<double> zp((size_t)N3), zd((size_t)N3);
+  orc_fft_c2r(N, np.data(), zp.data());
+  orc_fft_c2r(N, nd.data(), zd.data());
+  const double a15 = pow(a_t, 1.5);
+  for (i64 i = 0; i < N3; ++i) {
+    const double p = zp[i] / (N * N * N), d = zd[i] / (N * N * N);
+    out[i] = p / a15;
+    out[N3 + i] = d / a15 - 1.5 * H_t * p / a15;
+  }
+}

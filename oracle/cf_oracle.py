@@ -97,6 +97,9 @@ def lib():
                                                C.c_double, C.c_double]
         L.orc_cutoff_fouriers.argtypes = [C.c_int64, C.c_int64, _dp, _dp]
         L.orc_curvature_energy_density_cuda.argtypes = [C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double]
+        L.orc_wkb_tables.argtypes = [C.c_int64] + [C.c_double] * 6 + [_dp] * 4
+        L.orc_wkb_init.argtypes = [C.c_int64, _dp, C.c_double, C.c_double, _dp]
+        L.orc_wkb_evaluate.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, C.c_double, C.c_double, _dp]
         L.orc_ci_pade.restype = C.c_double
         L.orc_set_threads(min(8, os.cpu_count() or 1))
         _LIB = L
@@ -379,6 +382,33 @@ def comoving_psi(G, R_fft, factor, H_t):
     then the unnormalised C2R of each (equations.cpp:232-233)."""
     pk, dk = comoving_psi_fft(G, R_fft, factor, H_t)
     return fft_c2r(G, pk), fft_c2r(G, dk)
+
+
+def wkb_tables(N, L, m, a1, H1, t_i, t_f):
+    """cos(phase), sin(phase), sqrt(omega_eff_i), phase per s (wkb.cpp:40-96)."""
+    nb = nbins(N)
+    out = [np.empty(nb) for _ in range(4)]
+    lib().orc_wkb_tables(N, L, m, a1, H1, t_i, t_f, *[_p(o) for o in out])
+    return out
+
+
+class WKB:
+    """WKBSolutionForKleinGordonEquationInFRW (wkb.cpp:5-121) for a radiation-era
+    cosmology (p = 1, t1 = 1/(2 H1))."""
+
+    def __init__(self, N, L, m, cosmo, state, t_i):
+        assert cosmo.p == 1.0 and cosmo.t1 == 1.0 / (2 * cosmo.H1)
+        self.N, self.L, self.m, self.cosmo, self.t_i = N, L, m, cosmo, t_i
+        self.ffts = np.empty(4 * N * N * (N // 2 + 1))
+        lib().orc_wkb_init(N, _p(np.ascontiguousarray(state)), cosmo_a(cosmo, t_i), cosmo_H(cosmo, t_i),
+                           _p(self.ffts))
+
+    def evaluate_at(self, t):
+        c, s, w, _ = wkb_tables(self.N, self.L, self.m, self.cosmo.a1, self.cosmo.H1, self.t_i, t)
+        out = np.empty(2 * self.N ** 3)
+        lib().orc_wkb_evaluate(self.N, _p(self.ffts), _p(c), _p(s), _p(w), cosmo_a(self.cosmo, t),
+                               cosmo_H(self.cosmo, t), _p(out))
+        return out
 
 
 FIXED, SQRT, COMOVING, APPROX = 1, 2, 3, 4

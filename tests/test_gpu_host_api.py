@@ -48,7 +48,8 @@ def oracle_run(O, out, p, f32c=False):
         # only supplies a(t) and the lattice constants to the observer
         eq = O.Equation(N, p["L"], p["m"], 0.0, cosmo)
         v = O.VariantEquation(O.SQRT, N, p["L"], p["m"], cosmo, f_a=p["lam"])
-        obs = O.ConstIntervalObserver(eq, p["t0"], p["t1"], p["interval"], rho_fn=v.energy_density)
+        obs = O.ConstIntervalObserver(eq, p["t0"], p["t1"], p["interval"], save_density=True,
+                                      rho_fn=v.energy_density)
 
         def vstep(xx, t, dt):
             xx[:] = v.rk4_step(xx, t, dt)
@@ -56,10 +57,13 @@ def oracle_run(O, out, p, f32c=False):
         O.integrate_const(vstep, obs, x, p["t0"], p["t1"], p["dt"])
         return x, obs
     eq = O.Equation(N, p["L"], p["m"], p["lam"], cosmo)
-    obs = O.ConstIntervalObserver(eq, p["t0"], p["t1"], p["interval"])
+    obs = O.ConstIntervalObserver(eq, p["t0"], p["t1"], p["interval"], save_density=True)
     if f32c:
+        # the driver splits the state once and observes hi + lo from then on,
+        # including at t_start
         hi = x.astype(np.float32)
         lo = (x - hi.astype(np.float64)).astype(np.float32)
+        x[:] = hi.astype(np.float64) + lo.astype(np.float64)
 
         def step(_, t, dt):
             eq.rk4_step_f32c(hi, lo, t, dt)
@@ -84,6 +88,14 @@ def test_kg_run_matches_oracle(O, tmp_path, stepper, eq, lam):
     for i in range(len(obs.t_list)):
         spec_close(np.fromfile(out + f"varphi_plus_spectrum_{i}.dat"), obs.field_spectra[i])
         spec_close(np.fromfile(out + f"rho_spectrum_{i}.dat"), obs.rho_spectra[i])
+        check_density_files(out, obs, i)
+
+
+def check_density_files(out, obs, i):
+    """rho slice at a = 0 and the average over a: bit for bit (the device average
+    sums over a in order and divides once, as the reference's rowwise mean)."""
+    assert np.array_equal(np.fromfile(out + f"rho_slice_{i}.dat"), obs.rho_slices[i])
+    assert np.array_equal(np.fromfile(out + f"rho_axis_average_{i}.dat"), obs.rho_axis_averages[i])
 
 
 @pytest.mark.skipif(not os.path.exists(DROPIN), reason="built only where the reference tree is mounted")
@@ -103,3 +115,79 @@ def test_boost_odeint_dropin(O, tmp_path, lam):
     assert int(r.stdout.strip()) == n
     assert np.array_equal(np.fromfile(str(tmp_path / "state.bin")), x)
     np.testing.assert_array_equal(np.fromfile(str(tmp_path / "times.bin")), np.array(times))
+
+
+def _check_run(O, out, p, n_saves):
+    x, obs = oracle_run(O, out, p)
+    g = np.fromfile(out + "state.dat")
+    n = p["N"] ** 3
+    assert np.array_equal(g, x), (f"final state differs in {np.count_nonzero(g != x)} entries; rel L2 phi "
+                                  f"{np.linalg.norm(g[:n] - x[:n]) / np.linalg.norm(x[:n]):.3e}")
+    np.testing.assert_array_equal(np.fromfile(out + "t_list.dat"), np.array(obs.t_list))
+    assert len(obs.t_list) == n_saves
+    for i in range(len(obs.t_list)):
+        spec_close(np.fromfile(out + f"varphi_plus_spectrum_{i}.dat"), obs.field_spectra[i])
+        spec_close(np.fromfile(out + f"rho_spectrum_{i}.dat"), obs.rho_spectra[i])
+        check_density_files(out, obs, i)
+
+
+def test_config_kg_n64_minkowski(O, tmp_path):
+    """Plain Klein-Gordon, N=64, L=64, m=1, seed 0, P ~ exp(-k^2/(2*0.5^2)), rms 1,
+    dt=0.1 from t=0 to 10, spectra every 1.0 (11 saves): the fused HIP run equals
+    the oracle's restatement of the reference CPU run bit for bit."""
+    out, p = run_kg(tmp_path, "fused", "kg", N=64, L=64.0, m=1.0, t0=0.0, t1=10.0, interval=0.999, dt=0.1,
+                    seed=0, kind=0, sp=0.5, rms=1.0)
+    _check_run(O, out, p, 11)
+
+
+def test_config_frw_n256_oracle(O, tmp_path):
+    """KG in FRW (radiation era), N=256, L=256, m=1, seed 1, P ~ k^-3 exp(-k^2/2),
+    rms 0.1, dt=0.05 from t=1: the FP64 HIP drop-in run of the first 40 steps with
+    saves every 20 equals the oracle bit for bit (state) and to 1e-10 (spectra).
+    The oracle needs ~0.4 s per step at 256^3, so the full 2000 steps are checked
+    on the GPU by the test below."""
+    out, p = run_kg(tmp_path, "fused", "frw", N=256, L=256.0, m=1.0, t0=1.0, t1=3.0, interval=0.999,
+                    dt=0.05, seed=1, kind=1, sp=0.0, rms=0.1)
+    print("GPU run done; oracle run (40 steps at 256^3) follows", flush=True)
+    _check_run(O, out, p, 3)
+
+
+def test_config_frw_n256_2000_steps(cfl):
+    """The same workload for all 2000 steps, states compared every 100 steps between
+    three independent GPU evaluations of the reference's step: the odeint flow
+    (sys() + scale_sum2/scale_sum5, as the reference's GPU path runs it), the four
+    fused stage kernels, and the temporally blocked one-kernel step."""
+    import torch
+    N, L, dt, t0 = 256, 256.0, 0.05, 1.0
+    cosmo = cfl.radiation(1.0, 0.5, 1.0)
+    p = cfl.kg_params(N, L, 1.0, 0.0, cosmo)
+    plan = cfl.SpectrumPlan(N)
+    x0 = plan.init_grf_state(1, cfl.CF_SPEC_KM3_GAUSS, (), L, 0.1)
+    xf, xt, xc = x0.clone(), x0.clone(), x0.clone()
+    wf, wt = cfl.alloc_work(N), cfl.alloc_tb_work(N)
+    k = [torch.empty_like(x0) for _ in range(4)]
+    y = torch.empty_like(x0)
+
+    def f(xx, tt, out):
+        a = cfl.cosmo_a(cosmo, tt)
+        cfl.rhs_f64(N, xx, out, 1.0, 0.0, cfl.cosmo_H(cosmo, tt), cfl.inv_ah_sqr(N, L, a))
+
+    for block in range(20):
+        for j in range(100):
+            t = t0 + float(block * 100 + j) * dt
+            dh = dt / 2.0
+            f(xc, t, k[0])
+            cfl.scale_sum(y, [xc, k[0]], [1.0, dh])
+            f(y, t + dh, k[1])
+            cfl.scale_sum(y, [xc, k[1]], [1.0, dh])
+            f(y, t + dh, k[2])
+            cfl.scale_sum(y, [xc, k[2]], [1.0, dt])
+            f(y, t + dt, k[3])
+            cfl.scale_sum(xc, [xc] + k, [1.0, dt / 6, dt / 3, dt / 3, dt / 6])
+        cfl.rk4_run_f64(p, xf, t0, block * 100, 100, dt, wf)
+        cfl.rk4_run_tb_f64(p, xt, t0, block * 100, 100, dt, wt)
+        torch.cuda.synchronize()
+        assert torch.equal(xf, xc), f"fused vs odeint flow differ after {(block + 1) * 100} steps"
+        assert torch.equal(xt, xc), f"temporally blocked vs odeint flow differ after {(block + 1) * 100} steps"
+        print(f"step {(block + 1) * 100}: identical", flush=True)
+    assert torch.isfinite(xc).all()

@@ -49,11 +49,10 @@ def close(ours, ref, what, tol=1e-10, floor=1e-18):
     return float(np.median(rel))
 
 
-@pytest.fixture(scope="module")
-def run(O):
+def oracle_initial_run(O, P):
     """Restates the reference run's initialisation (initializer.hpp:150-206) and
-    its first observation (observer.hpp:87-124 at t = t_start)."""
-    P = read_param()
+    its first observation (observer.hpp:87-124 at t = t_start) for the parameter
+    dict P (the fields of main.cpp:36-64); seed 0."""
     N, L = P["N"], P["L"]
     pi = math.pi
     cosmo = O.Cosmo.radiation(P["a1"], P["H1"], P["t1"])
@@ -92,13 +91,14 @@ def run(O):
     dphi = O.grf_from_normals(N, nrm[2 * n3:], Pdf, alpha_dphi * Psi)
     del nrm
     R_fft = O.fft_r2c(N, R)
-    out = {"P": P, "N": N, "L": L, "cosmo": cosmo, "eta_i": eta_i}
+    out = {"P": P, "N": N, "L": L, "cosmo": cosmo, "eta_i": eta_i, "R_fft": R_fft}
     # main.cpp:164-176: R round-tripped through z2d/N^3, Psi recomputed from it
     R2 = O.fft_c2r(N, R_fft) / float(N ** 3)
     out["R_spectrum"] = O.power_spectrum(N, R2)
     out["Psi_spectrum"] = O.power_spectrum(N, O.scaled_fourier_modes(N, R2, kern))
     state = np.concatenate([phi, dphi])
     del phi, dphi, R, R2, Psi
+    out["state"] = state
     a_t = O.cosmo_a(cosmo, t0)
     out["field_spectrum"] = O.mode_power_spectrum(N, L, P["m"], a_t, state)
     # ComovingCurvatureEquationInFRW::compute_energy_density (equations.cpp:262-322)
@@ -111,6 +111,11 @@ def run(O):
     out["rho_slice"] = rho[: N * N].copy()
     out["rho_axis_average"] = rho.reshape(N, N * N).mean(axis=0)
     return out
+
+
+@pytest.fixture(scope="module")
+def run(O):
+    return oracle_initial_run(O, read_param())
 
 
 def test_param_file(run):
