@@ -108,8 +108,12 @@ cf_status cf_wkb_tables_host(int64_t N, double L, double m, double a1, double H1
                          (k_sqr * log(D(t_f) / D(t_i))) / (4. * pow(a1, 2) * H1 * m);
     // omega_eff at t_i (wkb.cpp:96)
     const double omega_eff_i = sqrt(m * m + k_sqr / (2 * a1 * a1 * H1 * t_i) + 3 / (16 * t_i * t_i));
-    cos_t[s] = std::cos(phase);
-    sin_t[s] = std::sin(phase);
+    // cos and sin of one argument: g++ -O3 (the reference's compiler) turns the
+    // pair into one sincos call, whose last bit can differ from separate calls
+    double sv, cv;
+    ::sincos(phase, &sv, &cv);
+    cos_t[s] = cv;
+    sin_t[s] = sv;
     sqw_t[s] = sqrt(omega_eff_i);
     if (phase_t) phase_t[s] = phase;
   }

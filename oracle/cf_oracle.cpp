@@ -856,8 +856,12 @@ extern "C" void orc_wkb_tables(i64 N, double L, double m, double a1, double H1, 
                          (std::sqrt(3) * std::log((1 + B(t_f)) / (1 + B(t_i)))) / 8. +
                          (k_sqr * std::log(D(t_f) / D(t_i))) / (4. * std::pow(a1, 2) * H1 * m);
     const double omega_eff_i = std::sqrt(m * m + k_sqr / (2 * a1 * a1 * H1 * t_i) + 3 / (16 * t_i * t_i));
-    cos_t[s] = cos(phase);
-    sin_t[s] = sin(phase);
+    // wkb.cpp:100-101 computes cos(phase) and sin(phase); g++ -O3 merges the pair
+    // into one sincos call, so the restatement calls sincos
+    double sv, cv;
+    ::sincos(phase, &sv, &cv);
+    cos_t[s] = cv;
+    sin_t[s] = sv;
     sqw_t[s] = std::sqrt(omega_eff_i);
     if (phase_t) phase_t[s] = phase;
   }

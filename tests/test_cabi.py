@@ -91,3 +91,16 @@ def test_bad_arguments_rejected_before_device_work(lib):
     assert L.cf_spectrum_table_host(9, 8, 1.0, None, C.c_void_p(0)) == lib.CF_E_INVALID
     with pytest.raises(lib.CfError):
         lib.check(L.cf_rhs_f64(5, fake, fake, 1, 0, 0, 1, None))
+
+
+@pytest.mark.parametrize("t_f", [10.0, 37.5, 160.0])
+def test_wkb_tables_equal_oracle(lib, O, t_f):
+    """cf_wkb_tables_host (the phase and omega tables of the WKB extrapolation, host
+    code) equals the oracle's restatement bit for bit."""
+    L = lib.load()
+    N, Lb, m, a1, H1, t_i = 24, 19.2, 1.0, 1.0, 0.05, 10.0
+    nb = 3 * (N // 2) ** 2 + 1
+    g = [np.empty(nb) for _ in range(4)]
+    lib.check(L.cf_wkb_tables_host(N, Lb, m, a1, H1, t_i, t_f, *[x.ctypes.data_as(C.c_void_p) for x in g]))
+    for a, b in zip(g, O.wkb_tables(N, Lb, m, a1, H1, t_i, t_f)):
+        assert np.array_equal(a, b)
