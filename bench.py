@@ -20,6 +20,7 @@ host on a bounded sample.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu] [--paths ...]
     python bench.py --workload precision --steps 1000     # run A vs run B errors
     python bench.py --workload ensemble --steps 500       # 96 uneven-N realisations
+    python bench.py --workload adjoint                    # loss + gradient, N=128, 1024 steps
 """
 import argparse
 import json
@@ -77,9 +78,10 @@ def parse():
                     help="headline: kernels to time (tb_f32c, tb_f64: one temporally blocked kernel per step; "
                          "f32c, f64: four fused stage kernels per step)")
     ap.add_argument("--save-dir", default="", help="ensemble: where to save the averaged density spectra (.npy)")
-    ap.add_argument("--workload", choices=("headline", "ensemble", "precision"), default="headline",
+    ap.add_argument("--workload", choices=("headline", "ensemble", "precision", "adjoint"), default="headline",
                     help="headline: N=512 FRW, FP32-comp vs FP64; ensemble: 96 realisations of N=64/96/128; "
-                         "precision: run A (FP32-comp) vs run B (FP64) errors every 100 steps")
+                         "precision: run A (FP32-comp) vs run B (FP64) errors every 100 steps; "
+                         "adjoint: loss + gradient through 1024 steps at N=128")
     ap.add_argument("--report-every", type=int, default=100, help="precision: steps between error reports")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run 3 device-to-device copies of 1 GiB (cf_d2d), the known-size "
@@ -173,6 +175,34 @@ def ensemble_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def observer_timing(cfl, plan, x, N, L, m, cosmo, t, stream, reps=5):
+    """One observation of ConstIntervalObserver (observer.hpp:87-124) on the
+    device: the field (mode) spectrum, the energy density and its spectrum, the
+    density slice and axis average. Milliseconds per observation (HIP events)."""
+    import torch
+    a_t = cfl.cosmo_a(cosmo, t)
+    rho = torch.empty(N ** 3, dtype=torch.float64, device="cuda")
+    avg = torch.empty(N * N, dtype=torch.float64, device="cuda")
+
+    def observe():
+        plan.mode_power_spectrum(x, L, m, a_t)
+        cfl.energy_density_f64(N, x, rho, m, 0.0, cfl.inv_ah_sqr(N, L, a_t))
+        plan.power_spectrum(rho)
+        cfl.axis_average(N, rho, avg)
+
+    observe()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(reps):
+        observe()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return {"ms_per_observation": ev0.elapsed_time(ev1) / reps,
+            "what": "mode spectrum (2 D2Z + binning) + energy density + density spectrum (D2Z + binning) "
+                    "+ density axis average, FP64"}
+
+
 def precision_workload(args):
     """Run A (FP32 hi + FP32 compensation, FP64 arithmetic) and run B (FP64) of the
     N=512 FRW workload from the same initial state (seed 2, P(k) ~ exp(-k^2/(2*0.25^2)),
@@ -249,8 +279,53 @@ def precision_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def adjoint_workload(args):
+    """Loss L = sum_s w_s P_rho(s, t_end), w = 1/(1+k^2), and its gradient with
+    respect to phi(t0) and m through 1024 FP64 RK4 steps of KG-FRW at N=128, L=128,
+    m=1, dt=0.05 (seed 3, rms 0.3), checkpoints every 32 steps: the forward pass,
+    the loss adjoint and the checkpointed backward pass (each segment recomputed).
+    Reported as wall time per gradient and as forward-equivalent site-steps/s."""
+    import torch
+    import cfl_amd as cfl
+    from cfl_amd.adjoint import AdjointProblem
+    N, L, m, t0, dt, K, ck = 128, 128.0, 1.0, 1.0, 0.05, 1024, 32
+    plan = cfl.SpectrumPlan(N)
+    phi0 = plan.init_grf_state(3, cfl.CF_SPEC_GAUSS, (0.5,), L, 0.3)[: N ** 3].clone()
+    del plan
+    prob = AdjointProblem(N, L, 0.0, cfl.radiation(1.0, 0.5, 1.0), t0, dt, K, ck)
+    prob.loss_and_grads(phi0, m)  # warm-up (plans, first launches)
+    torch.cuda.synchronize()
+    reps = max(1, args.steps // 100)
+    w0 = time.perf_counter()
+    for _ in range(reps):
+        Lv, g, dm = prob.loss_and_grads(phi0, m)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - w0) / reps
+    w1 = time.perf_counter()
+    prob.loss(phi0, m)
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - w1
+    line = {
+        "metric": "loss + gradient wall time, spectral loss through 1024 RK4 steps, N=128^3 FP64",
+        "value": sec, "unit": "s per gradient", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
+        "data": "synthetic (Gaussian random field, seed 3)",
+        "config": {"workload": f"KG-FRW N={N}^3 L={L:g} m=1 dt=0.05, {K} steps, checkpoints every {ck}",
+                   "repetitions": reps},
+        "forward_only_s": fwd, "gradient_over_forward": sec / fwd,
+        "forward_equivalent_gsite_steps": N ** 3 * K / sec / 1e9,
+        "loss": Lv, "dL_dm": dm,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
+    if args.workload == "adjoint":
+        if args.gpus != 1:
+            print("bench.py: this build runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        adjoint_workload(args)
+        return
     if args.workload == "precision":
         if args.gpus != 1:
             print("bench.py: this build runs on one GPU only", file=sys.stderr)
@@ -285,6 +360,7 @@ def main():
     plan = cfl.SpectrumPlan(N)
     x0 = plan.init_grf_state(2, cfl.CF_SPEC_GAUSS, (0.25,), L, 1.0)
     torch.cuda.synchronize()
+    observer = observer_timing(cfl, plan, x0, N, L, m, cosmo, t0, stream)
     del plan
 
     paths = [q for q in args.paths.split(",") if q]
@@ -357,6 +433,7 @@ def main():
                                f"(run A FP32-comp, run B FP64)", "N": N, "L": L, "dt": dt,
                    "headline_path": head_mode, "parallelism": "single GPU"},
         "paths": results,
+        "observer": observer,
         "roofline": {"bound": "hbm", "kernel": head["kernel"],
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

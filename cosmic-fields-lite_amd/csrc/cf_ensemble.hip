@@ -74,6 +74,7 @@ __global__ __launch_bounds__(1024) void k_ens_tb(const EnsReal *__restrict__ rea
     TbStep k;
     // The step scalars are the same for the whole block; moving them to scalar
     // registers keeps the vector registers for the step body.
+#pragma unroll
     for (int s = 0; s < 4; ++s) {
       k.c_h[s] = wave_uniform((-3.0) * sc.H[s]);
       k.inv[s] = wave_uniform(1.0 / ((R.L / R.N) * (R.L / R.N)) / (sc.a[s] * sc.a[s]));  // cf_inv_ah_sqr

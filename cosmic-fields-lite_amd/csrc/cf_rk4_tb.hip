@@ -42,7 +42,9 @@ __global__ __launch_bounds__(1024) void k_rk4_tb(TbGeom g, TbStep k, const T *__
   u.c0 = (int64_t)tc * g.T;
   u.a0 = (int64_t)ac * g.A;
   u.a1 = (u.a0 + g.A < g.N) ? u.a0 + g.A : g.N;
-  tb_unit<T, F32C, LAM>(lds, u, k, xin, lin, xout, lout);
+  // Lane shifts for the c neighbours in FP64; LDS in the compensated mode, whose
+  // float plane buffers are cheap and whose VALU load is the larger (measured).
+  tb_unit<T, F32C, LAM, true, !F32C>(lds, u, k, xin, lin, xout, lout);
 }
 
 static TbGeom tb_geom(int64_t N, int A) {

@@ -15,8 +15,10 @@
 //     stage 3 at P-3 (needs y2 at P-4..P-2)    -> y3, acc
 //     stage 4 at P-4 (needs y3 at P-5..P-3)    -> new state at P-4 (stored for j >= 8)
 //
-// so a chunk of A planes takes A + 8 iterations. The in-plane neighbours
-// (b+-1, c+-1) of each stage input come from plane buffers in LDS. The four
+// so a chunk of A planes takes A + 8 iterations. The b+-1 neighbours of each
+// stage input come from plane buffers in LDS (other wavefronts hold those rows),
+// the c+-1 neighbours from the adjacent lanes of the same wavefront (DPP wave
+// shifts) or, with LANE_NBR = false, from the same LDS buffers. The four
 // stage inputs of an iteration are all known when it starts (the a+1 neighbour
 // each stage needs from the current iteration is in registers), so they are
 // written together and one barrier per iteration suffices. The region shrinks by
@@ -88,6 +90,35 @@ __device__ __forceinline__ float tb_load_f32(__amdgpu_buffer_rsrc_t r, uint32_t 
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
 }
 
+// The c+1 and c-1 neighbours of a stage input are the values of the lanes to the
+// right and left in the same wavefront (a wavefront holds two rows of 32 columns),
+// moved with DPP wave shifts instead of read back from LDS. The lanes at a row's
+// ends receive a value from the other row (or 0); they are halo columns whose
+// results are never used, exactly as with clamped LDS indices.
+template <int CTRL>
+__device__ __forceinline__ int tb_dpp(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL, class V>
+__device__ __forceinline__ V tb_lane_shift(V v) {
+  if constexpr (sizeof(V) == 4) {
+    return __builtin_bit_cast(V, tb_dpp<CTRL>(__builtin_bit_cast(int, v)));
+  } else {
+    int2 w = __builtin_bit_cast(int2, v);
+    w.x = tb_dpp<CTRL>(w.x);
+    w.y = tb_dpp<CTRL>(w.y);
+    return __builtin_bit_cast(V, w);
+  }
+}
+template <class V>
+__device__ __forceinline__ V tb_lane_next(V v) {  // lane + 1 (DPP wave_shl:1)
+  return tb_lane_shift<0x130>(v);
+}
+template <class V>
+__device__ __forceinline__ V tb_lane_prev(V v) {  // lane - 1 (DPP wave_shr:1)
+  return tb_lane_shift<0x138>(v);
+}
+
 // The new state of one iteration, stored two iterations later.
 template <class T>
 struct TbPending {
@@ -105,13 +136,18 @@ struct TbPending {
 template <class T>
 using TbLds = T[2][4][TBR][TBR];
 
-template <class T, bool F32C, bool LAM, bool DEFER = true>
+template <class T, bool F32C, bool LAM, bool DEFER = true, bool LANE_NBR = true>
 __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const TbStep &k, const T *__restrict__ xin,
                                         const float *__restrict__ lin, T *__restrict__ xout,
                                         float *__restrict__ lout) {
   const int tid = threadIdx.x;
   const int r = tid >> 5, cc = tid & 31;
   static_assert(!F32C || sizeof(T) == sizeof(float), "the compensated mode stores float");
+  // In the compensated mode every value carried from plane to plane is exactly
+  // representable in float (the state hi, the rounded stage inputs and the rounded
+  // increment), so the histories are held as float (half the registers) and
+  // widened where they are used; the arithmetic stays double.
+  typedef std::conditional_t<F32C, float, double> Hv;
   const int64_t N = u.N, NN = N * N, N3 = NN * N;
   const int plane_bytes = (int)(NN * (int64_t)sizeof(T)), plane_bytes_f = (int)(NN * (int64_t)sizeof(float));
   int64_t b = (u.b0 - TBH + r) % N;
@@ -123,6 +159,17 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
                       u.c0 + (cc - TBH) < N;
   const int rm = r > 0 ? r - 1 : 0, rp = r < TBR - 1 ? r + 1 : TBR - 1;
   const int cm = cc > 0 ? cc - 1 : 0, cp = cc < TBR - 1 ? cc + 1 : TBR - 1;
+  // c+1 / c-1 neighbour of a stage input: from the adjacent lanes (LANE_NBR) or
+  // from the LDS plane buffer (the same values; which is faster depends on the
+  // kernel's mix of LDS and VALU work)
+  auto cnext = [&](Hv v, const T(*pl)[TBR]) -> double {
+    if constexpr (LANE_NBR) return (double)tb_lane_next(v);
+    else return (double)pl[r][cp];
+  };
+  auto cprev = [&](Hv v, const T(*pl)[TBR]) -> double {
+    if constexpr (LANE_NBR) return (double)tb_lane_prev(v);
+    else return (double)pl[r][cm];
+  };
 
   // plane offsets (times N^2) advanced by one plane per iteration, wrapping at N
   auto wrap_plane = [&](int64_t P) -> int64_t {
@@ -135,11 +182,6 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     return off == N3 ? 0 : off;
   };
 
-  // In the compensated mode every value carried from plane to plane is exactly
-  // representable in float (the state hi, the rounded stage inputs and the rounded
-  // increment), so the histories are held as float (half the registers) and
-  // widened where they are used; the arithmetic stays double.
-  typedef std::conditional_t<F32C, float, double> Hv;
   Hv xf0 = 0, xf1 = 0, xf2 = 0, xf3 = 0, xf4 = 0;  // x phi at P, P-1, ..., P-4
   Hv xp0 = 0, xp1 = 0, xp2 = 0, xp3 = 0, xp4 = 0;  // x pi
   Hv y1f1 = 0, y1f2 = 0, y1p1 = 0;                  // y1 phi at P-2, P-3 (start of iteration); pi at P-2
@@ -217,7 +259,7 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     // ---- stage 1 at P-1: k1 = f(x)
     double kf = (double)xp1;
     double kp = tb_rhs((double)xf1, (double)xf0, (double)xf2, (double)buf[0][rp][cc], (double)buf[0][rm][cc],
-                       (double)buf[0][r][cp], (double)buf[0][r][cm], (double)xp1, k.c_h[0], k.c_m, k.lam,
+                       cnext(xf1, buf[0]), cprev(xf1, buf[0]), (double)xp1, k.c_h[0], k.c_m, k.lam,
                        k.inv[0], LAM);
     const Hv y1f0 = (Hv)rnd<F32C>((double)xf1 + k.dh * kf);
     const Hv y1p0 = (Hv)rnd<F32C>((double)xp1 + k.dh * kp);
@@ -233,7 +275,7 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     // ---- stage 2 at P-2: k2 = f(y1)
     kf = (double)y1p1;
     kp = tb_rhs((double)y1f1, (double)y1f0, (double)y1f2, (double)buf[1][rp][cc], (double)buf[1][rm][cc],
-                (double)buf[1][r][cp], (double)buf[1][r][cm], (double)y1p1, k.c_h[1], k.c_m, k.lam, k.inv[1],
+                cnext(y1f1, buf[1]), cprev(y1f1, buf[1]), (double)y1p1, k.c_h[1], k.c_m, k.lam, k.inv[1],
                 LAM);
     const Hv y2f0 = (Hv)rnd<F32C>((double)xf2 + k.dh * kf);
     const Hv y2p0 = (Hv)rnd<F32C>((double)xp2 + k.dh * kp);
@@ -243,7 +285,7 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     // ---- stage 3 at P-3: k3 = f(y2)
     kf = (double)y2p1;
     kp = tb_rhs((double)y2f1, (double)y2f0, (double)y2f2, (double)buf[2][rp][cc], (double)buf[2][rm][cc],
-                (double)buf[2][r][cp], (double)buf[2][r][cm], (double)y2p1, k.c_h[2], k.c_m, k.lam, k.inv[2],
+                cnext(y2f1, buf[2]), cprev(y2f1, buf[2]), (double)y2p1, k.c_h[2], k.c_m, k.lam, k.inv[2],
                 LAM);
     const Hv y3f0 = (Hv)rnd<F32C>((double)xf3 + k.dt * kf);
     const Hv y3p0 = (Hv)rnd<F32C>((double)xp3 + k.dt * kp);
@@ -253,7 +295,7 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     // ---- stage 4 at P-4: k4 = f(y3); new state
     kf = (double)y3p1;
     kp = tb_rhs((double)y3f1, (double)y3f0, (double)y3f2, (double)buf[3][rp][cc], (double)buf[3][rm][cc],
-                (double)buf[3][r][cp], (double)buf[3][r][cm], (double)y3p1, k.c_h[3], k.c_m, k.lam, k.inv[3],
+                cnext(y3f1, buf[3]), cprev(y3f1, buf[3]), (double)y3p1, k.c_h[3], k.c_m, k.lam, k.inv[3],
                 LAM);
     if constexpr (F32C) {
       const double s0 = ((double)xf4 + lo_f) + ((double)af3 + k.dt6 * kf);

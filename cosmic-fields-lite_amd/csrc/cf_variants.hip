@@ -31,7 +31,6 @@
 #include <hipfft/hipfft.h>
 
 #include <cmath>
-#include <cstring>
 
 #include "cf_march.hpp"
 

@@ -19,8 +19,6 @@
 
 #include "cf_common.hpp"
 
-struct cf_spectrum_plan;
-
 namespace cf {
 
 // Per-entry passes over the half spectrum (N*N*(N/2+1) complex values).

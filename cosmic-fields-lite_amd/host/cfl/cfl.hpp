@@ -18,3 +18,4 @@
 #include "physics.hpp"
 #include "stepper.hpp"
 #include "workspace.hpp"
+#include "wkb.hpp"

@@ -51,8 +51,11 @@ inline std::vector<double> to_host(const hip_vector<double> &d) {
   return h;
 }
 
+// save_radial adds N/2 radial bins (bin j sums the s-bins with floor(sqrt(s)) = j)
+// of each saved spectrum as varphi_plus_radial_%d.dat / rho_radial_%d.dat; the
+// reference writes the s-bins only.
 template <typename Equation, bool save_field_spectrum = true, bool save_density_spectrum = true,
-          bool save_density = false>
+          bool save_density = false, bool save_radial = false>
 struct ConstIntervalObserver {
   typedef typename Equation::Workspace Workspace;
   typedef typename Workspace::State State;
@@ -82,12 +85,18 @@ struct ConstIntervalObserver {
     if constexpr (save_field_spectrum) {
       hip_vector<double> s = compute_mode_power_spectrum(N, L, m, a_t, workspace.state, workspace.fft_wrapper);
       write_vector_to_filename_template(to_host(s), dir + "varphi_plus_spectrum_%d.dat", idx);
+      if constexpr (save_radial)
+        write_vector_to_filename_template(to_host(compute_radial_spectrum(N, s, workspace.fft_wrapper)),
+                                          dir + "varphi_plus_radial_%d.dat", idx);
     }
     if constexpr (save_density_spectrum || save_density) {
       hip_vector<double> rho = Equation::compute_energy_density(workspace, t);
       if constexpr (save_density_spectrum) {
         hip_vector<double> s = compute_power_spectrum(N, rho, workspace.fft_wrapper);
         write_vector_to_filename_template(to_host(s), dir + "rho_spectrum_%d.dat", idx);
+        if constexpr (save_radial)
+          write_vector_to_filename_template(to_host(compute_radial_spectrum(N, s, workspace.fft_wrapper)),
+                                            dir + "rho_radial_%d.dat", idx);
       }
       if constexpr (save_density) {
         // slice at a = 0 and the average over a (observer.hpp:110-119), both

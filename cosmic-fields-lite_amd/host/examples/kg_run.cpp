@@ -8,11 +8,14 @@
 //             fused   (cfl::hip_runge_kutta4: fused stage kernels)
 //             f32c    (cfl::hip_runge_kutta4_f32c: FP32-compensated storage)
 //   equation: kg | frw | lambda | sqrt (sqrt potential; the lambda argument is f_a)
-// Writes state0.dat (initial state), the observer's spectra and density slices, t_list.dat
-// and state.dat.
+// Writes state0.dat (initial state), the observer's spectra (s-bins and N/2 radial bins)
+// and density slices, t_list.dat
+// and state.dat; for frw in a radiation era with t1 = 1/(2 H1), also wkb_state.dat (the
+// final state extrapolated by WKB to t_end + 10).
 #include <cstdio>
 #include <cstdlib>
 #include <string>
+#include <type_traits>
 
 #include "cfl/cfl.hpp"
 
@@ -38,7 +41,7 @@ static int run(const std::string &dir, const std::string &stepper, const RunPara
   Workspace workspace(param, init);
   cfl::write_vector_to_file(cfl::to_host(workspace.state), dir + "state0.dat");
   Equation eqn(workspace);
-  cfl::ConstIntervalObserver<Equation, true, true, true> observer(dir, param, eqn);
+  cfl::ConstIntervalObserver<Equation, true, true, true, true> observer(dir, param, eqn);
   std::size_t steps = 0;
   if (stepper == "classic") {
     steps = cfl::odeint_lite::integrate_const(cfl::odeint_lite::runge_kutta4_classic<>(), eqn, workspace.state,
@@ -65,6 +68,14 @@ static int run(const std::string &dir, const std::string &stepper, const RunPara
   }
   cfl::write_vector_to_file(workspace.t_list, dir + "t_list.dat");
   cfl::write_vector_to_file(cfl::to_host(workspace.state), dir + "state.dat");
+  if constexpr (std::is_same_v<Equation, cfl::HipKleinGordonEquationInFRW>) {
+    // WKB extrapolation of the final state to t_end + 10 (generate_wkb_solutions,
+    // main.cpp:199-275), where the cosmology allows it (radiation era, t1 = 1/(2 H1))
+    if (param.H1 > 0 && param.t1 == 1.0 / (2 * param.H1) && param.m > 0) {
+      cfl::WKBSolutionForKleinGordonEquationInFRW wkb(workspace, param.t_end);
+      cfl::write_vector_to_file(cfl::to_host(wkb.evaluate_at(param.t_end + 10.0)), dir + "wkb_state.dat");
+    }
+  }
   std::printf("steps %zu saves %zu\n", steps, workspace.t_list.size());
   return 0;
 }

@@ -75,6 +75,18 @@ def oracle_run(O, out, p, f32c=False):
     return x, obs
 
 
+def test_wkb_extrapolation_of_final_state(O, tmp_path):
+    """cfl::WKBSolutionForKleinGordonEquationInFRW from the final state of an FRW run,
+    evaluated at t_end + 10, against the oracle's WKB from the same state (FFT
+    rounding apart: <= 1e-12 of the field's maximum)."""
+    out, p = run_kg(tmp_path, "fused", "frw")
+    x = np.fromfile(out + "state.dat")
+    w = O.WKB(p["N"], p["L"], p["m"], O.Cosmo.radiation(p["a1"], p["H1"], p["t1c"]), x, p["t1"])
+    ref = w.evaluate_at(p["t1"] + 10.0)
+    g = np.fromfile(out + "wkb_state.dat")
+    assert np.abs(g - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("stepper,eq,lam", [("classic", "frw", 0.0), ("fused", "frw", 0.0),
                                             ("fused", "lambda", 0.1), ("fused", "kg", 0.0),
                                             ("f32c", "frw", 0.0), ("classic", "sqrt", 0.7),
@@ -89,6 +101,19 @@ def test_kg_run_matches_oracle(O, tmp_path, stepper, eq, lam):
         spec_close(np.fromfile(out + f"varphi_plus_spectrum_{i}.dat"), obs.field_spectra[i])
         spec_close(np.fromfile(out + f"rho_spectrum_{i}.dat"), obs.rho_spectra[i])
         check_density_files(out, obs, i)
+        check_radial_files(out, obs, i, p["N"])
+
+
+def check_radial_files(out, obs, i, N):
+    """N/2 radial bins of each saved spectrum: sums of the s-bins in order, so they
+    follow the s-bins' own tolerance."""
+    spec_close(np.fromfile(out + f"varphi_plus_radial_{i}.dat"), O_radial(N, obs.field_spectra[i]))
+    spec_close(np.fromfile(out + f"rho_radial_{i}.dat"), O_radial(N, obs.rho_spectra[i]))
+
+
+def O_radial(N, s):
+    import cf_oracle
+    return cf_oracle.radial_bins(N, s)
 
 
 def check_density_files(out, obs, i):
@@ -129,6 +154,7 @@ def _check_run(O, out, p, n_saves):
         spec_close(np.fromfile(out + f"varphi_plus_spectrum_{i}.dat"), obs.field_spectra[i])
         spec_close(np.fromfile(out + f"rho_spectrum_{i}.dat"), obs.rho_spectra[i])
         check_density_files(out, obs, i)
+        check_radial_files(out, obs, i, p["N"])
 
 
 def test_config_kg_n64_minkowski(O, tmp_path):
