@@ -37,3 +37,39 @@ def test_evaluate_at_start_returns_state(O):
     assert np.abs(y - x).max() <= 1e-13 * np.abs(x).max()
     c, s, _, ph = O.wkb_tables(N, L, m, 1.0, 0.05, 10.0, 10.0)
     assert np.all(ph == 0.0) and np.all(c == 1.0) and np.all(s == 0.0)
+
+
+def test_rotation_preserves_mode_invariant(O):
+    """Per mode, evaluate_at rotates (phi^, pi^) by [[c, s/w], [-s w, c]] with
+    w = sqrt(omega_eff(t_i)): a determinant-one map that keeps
+    omega_eff(t_i) |phi^|^2 + |pi^|^2. Recovering the rotated modes from the output
+    state (undoing the a^1.5 and 1.5 H factors) must give the same invariant per
+    mode; a sign or factor slip in either row breaks it."""
+    N, L, m = 12, 9.6, 1.0
+    H1, t_i, t_f = 0.05, 10.0, 47.0
+    cosmo = O.Cosmo.radiation(1.0, H1, t_i)
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.standard_normal(N ** 3), 0.3 * rng.standard_normal(N ** 3)])
+    w = O.WKB(N, L, m, cosmo, x, t_i)
+    y = w.evaluate_at(t_f)
+    a15 = O.cosmo_a(cosmo, t_f) ** 1.5
+    H = O.cosmo_H(cosmo, t_f)
+    n3 = N ** 3
+    p_new = O.fft_r2c(N, y[:n3] * a15)
+    d_new = O.fft_r2c(N, y[n3:] * a15 + 1.5 * H * y[:n3] * a15)
+    fs = 2 * N * N * (N // 2 + 1)
+    p_old, d_old = w.ffts[:fs], w.ffts[fs:]
+    _, _, sqw, _ = O.wkb_tables(N, L, m, 1.0, H1, t_i, t_f)
+    a = np.arange(N)[:, None, None]
+    b = np.arange(N)[None, :, None]
+    c = np.arange(N // 2 + 1)[None, None, :]
+    s = (np.minimum(a, N - a) ** 2 + np.minimum(b, N - b) ** 2 + c ** 2).ravel()
+    om = sqw[s] ** 2
+
+    def inv(p, d):
+        pc = p.reshape(-1, 2)
+        dc = d.reshape(-1, 2)
+        return om * (pc ** 2).sum(1) + (dc ** 2).sum(1)
+
+    q0, q1 = inv(p_old, d_old), inv(p_new, d_new)
+    assert np.abs(q1 - q0).max() <= 1e-10 * q0.max()
