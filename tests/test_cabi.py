@@ -91,6 +91,26 @@ def test_bad_arguments_rejected_before_device_work(lib):
     assert L.cf_spectrum_table_host(9, 8, 1.0, None, C.c_void_p(0)) == lib.CF_E_INVALID
     with pytest.raises(lib.CfError):
         lib.check(L.cf_rhs_f64(5, fake, fake, 1, 0, 0, 1, None))
+    # equation variants, WKB, ensembles, cut-off, observer average
+    p = lib.KGParams(16, 16.0, 1.0, 0.0, lib.Cosmology(1.0, 0.5, 1.0, 1.0))
+    h = C.c_void_p()
+    assert L.cf_variant_create(C.byref(p), 9, 1.0, 0, fake, C.byref(h), None) == lib.CF_E_INVALID
+    assert L.cf_variant_create(C.byref(p), lib.CF_VAR_FIXED_CURVATURE, 1.0, 0, None, C.byref(h), None) == \
+        lib.CF_E_INVALID  # needs Psi
+    assert L.cf_variant_create(C.byref(p), lib.CF_VAR_SQRT_POTENTIAL, 0.0, 0, None, C.byref(h), None) == \
+        lib.CF_E_INVALID  # f_a must be positive
+    assert L.cf_variant_create(C.byref(p), lib.CF_VAR_APPROX_COMOVING_CURVATURE, 1.0, 6, fake, C.byref(h),
+                               None) == lib.CF_E_INVALID  # M must divide N
+    assert h.value is None
+    pw = lib.KGParams(16, 16.0, 1.0, 0.0, lib.Cosmology(1.0, 0.5, 3.0, 1.0))  # t1 != 1/(2 H1)
+    assert L.cf_wkb_create(C.byref(pw), fake, 3.0, C.byref(h), None) == lib.CF_E_INVALID
+    assert b"radiation" in L.cf_last_error()
+    assert L.cf_cutoff_fouriers(16, 32, fake, C.c_void_p(0x2000), None) == lib.CF_E_INVALID  # M > N
+    assert L.cf_axis_average(16, fake, fake, None) == lib.CF_E_INVALID  # aliased
+    Ns = (C.c_int64 * 2)(16, 7)
+    Ls = (C.c_double * 2)(16.0, 7.0)
+    assert L.cf_ensemble_create(2, Ns, Ls, 1.0, 0.0, C.byref(lib.Cosmology(1, 0, 0, 1)), 0, C.byref(h)) == \
+        lib.CF_E_INVALID  # odd N
 
 
 @pytest.mark.parametrize("t_f", [10.0, 37.5, 160.0])
