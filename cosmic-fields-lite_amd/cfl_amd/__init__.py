@@ -146,6 +146,16 @@ def rk4_run_tb_f32c(p, hi, lo, t0, k0, n_steps, dt, work, chunk=0, stream=None):
                                  _stream(stream)))
 
 
+def rk4_step_tb_f64(p, x_in, x_out, t, dt, chunk=0, stream=None):
+    """One temporally blocked step out of place: x_out = step(x_in)."""
+    check(lib.cf_rk4_step_tb_f64(C.byref(p), _ptr(x_in), _ptr(x_out), t, dt, chunk, _stream(stream)))
+
+
+def rk4_step_tb_f32c(p, hi_in, lo_in, hi_out, lo_out, t, dt, chunk=0, stream=None):
+    check(lib.cf_rk4_step_tb_f32c(C.byref(p), _ptr(hi_in), _ptr(lo_in), _ptr(hi_out), _ptr(lo_out), t, dt, chunk,
+                                  _stream(stream)))
+
+
 def split_f64(x, stream=None):
     hi = torch.empty(x.numel(), dtype=torch.float32, device=x.device)
     lo = torch.empty_like(hi)
@@ -356,6 +366,7 @@ class Variant:
         self.h, self.p, self.kind, self.M = h, p, kind, M
         self.N = p.N
         self._work = None
+        self._tb_work = None
 
     def close(self):
         if self.h:
@@ -384,6 +395,16 @@ class Variant:
 
     def rk4_run(self, x, t0, k0, n_steps, dt, stream=None):
         check(lib.cf_variant_rk4_run(self.h, _ptr(x), t0, k0, n_steps, dt, _ptr(self._w()), _stream(stream)))
+
+    def rk4_run_tb(self, x, t0, k0, n_steps, dt, chunk=0, stream=None):
+        """Temporally blocked steps (fixed curvature, sqrt potential); same bits as rk4_run."""
+        if self._tb_work is None:
+            self._tb_work = alloc_tb_work(self.N, CF_PREC_F64)
+        check(lib.cf_variant_rk4_run_tb(self.h, _ptr(x), t0, k0, n_steps, dt, chunk, _ptr(self._tb_work),
+                                        _stream(stream)))
+
+    def rk4_step_tb(self, x_in, x_out, t, dt, chunk=0, stream=None):
+        check(lib.cf_variant_rk4_step_tb(self.h, _ptr(x_in), _ptr(x_out), t, dt, chunk, _stream(stream)))
 
     def energy_density(self, x, t, out=None, stream=None):
         if out is None:

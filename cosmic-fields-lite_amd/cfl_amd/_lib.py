@@ -84,6 +84,8 @@ _SIGS = [
     ("cf_rk4_run_f32c", st, [C.POINTER(KGParams), vp, vp, dbl, i64, i64, dbl, vp, vp]),
     ("cf_rk4_tb_work_bytes", C.c_size_t, [i64, C.c_int]),
     ("cf_rk4_run_tb_f64", st, [C.POINTER(KGParams), vp, dbl, i64, i64, dbl, C.c_int, vp, vp]),
+    ("cf_rk4_step_tb_f64", st, [C.POINTER(KGParams), vp, vp, dbl, dbl, C.c_int, vp]),
+    ("cf_rk4_step_tb_f32c", st, [C.POINTER(KGParams), vp, vp, vp, vp, dbl, dbl, C.c_int, vp]),
     ("cf_rk4_run_tb_f32c", st, [C.POINTER(KGParams), vp, vp, dbl, i64, i64, dbl, C.c_int, vp, vp]),
     ("cf_ensemble_create", st, [C.c_int, vp, vp, dbl, dbl, C.POINTER(Cosmology), C.c_int, C.POINTER(vp)]),
     ("cf_ensemble_destroy", st, [vp]),
@@ -106,6 +108,8 @@ _SIGS = [
     ("cf_variant_rhs", st, [vp, vp, vp, dbl, vp]),
     ("cf_variant_rk4_step", st, [vp, vp, dbl, dbl, vp, vp]),
     ("cf_variant_rk4_run", st, [vp, vp, dbl, i64, i64, dbl, vp, vp]),
+    ("cf_variant_rk4_run_tb", st, [vp, vp, dbl, i64, i64, dbl, C.c_int, vp, vp]),
+    ("cf_variant_rk4_step_tb", st, [vp, vp, vp, dbl, dbl, C.c_int, vp]),
     ("cf_variant_energy_density", st, [vp, vp, vp, dbl, vp]),
     ("cf_variant_psi", st, [vp, dbl, vp, vp, vp]),
     ("cf_comoving_psi_fft", st, [i64, vp, vp, vp, dbl, dbl, vp]),

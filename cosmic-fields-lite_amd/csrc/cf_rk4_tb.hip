@@ -10,6 +10,7 @@
 #include <cmath>
 
 #include "cf_tb_body.hpp"
+#include "cf_tb_run.hpp"
 
 namespace cf {
 
@@ -47,6 +48,28 @@ __global__ __launch_bounds__(1024) void k_rk4_tb(TbGeom g, TbStep k, const T *__
   tb_unit<T, F32C, LAM, true, !F32C>(lds, u, k, xin, lin, xout, lout);
 }
 
+// The same step for the fixed-curvature (Psi, N^3) and sqrt-potential equation
+// variants, FP64.
+template <int VAR>
+__global__ __launch_bounds__(1024) void k_rk4_tb_var(TbGeom g, TbStep k, const double *__restrict__ xin,
+                                                     double *__restrict__ xout, const double *__restrict__ psi) {
+  __shared__ TbLds<double> lds;
+  int bid = blockIdx.x;
+  if ((g.total & 7) == 0) bid = (bid & 7) * (g.total >> 3) + (bid >> 3);
+  const int tc = bid % g.nt;
+  const int rest = bid / g.nt;
+  const int tb = rest % g.nt;
+  const int ac = rest / g.nt;
+  TbUnit u;
+  u.N = g.N;
+  u.T = g.T;
+  u.b0 = (int64_t)tb * g.T;
+  u.c0 = (int64_t)tc * g.T;
+  u.a0 = (int64_t)ac * g.A;
+  u.a1 = (u.a0 + g.A < g.N) ? u.a0 + g.A : g.N;
+  tb_unit<double, false, false, true, true, VAR>(lds, u, k, xin, nullptr, xout, nullptr, psi);
+}
+
 static TbGeom tb_geom(int64_t N, int A) {
   TbGeom g;
   g.N = N;
@@ -71,10 +94,50 @@ static cf_status tb_launch(const TbGeom &g, const TbStep &k, const T *xin, const
   return CF_OK;
 }
 
+// [a, a + na) and [b, b + nb) do not overlap.
+static bool disjoint(const void *a, size_t na, const void *b, size_t nb) {
+  const char *x = static_cast<const char *>(a), *y = static_cast<const char *>(b);
+  return x + na <= y || y + nb <= x;
+}
+static bool disjoint(const void *a, const void *b, size_t bytes) { return disjoint(a, bytes, b, bytes); }
+
 static cf_status tb_check(const cf_kg_params *p) {
   CF_REQUIRE(p, "null cf_kg_params");
   CF_REQUIRE(valid_N(p->N), "unsupported lattice size N=%lld (even, 4..2048)", (long long)p->N);
   CF_REQUIRE(p->L > 0.0, "box size L must be positive");
+  return CF_OK;
+}
+
+cf_status tb_step_variant(int var, const cf_kg_params *p, double f_a, const double *psi, const double *xin,
+                          double *xout, double t, double dt, int chunk, hipStream_t st) {
+  CF_TRY(tb_check(p));
+  CF_REQUIRE(var == TB_SQRT_POTENTIAL || (var == TB_FIXED_CURVATURE && psi), "tb_step_variant: bad variant");
+  const size_t n = 2 * (size_t)p->N * p->N * p->N;
+  CF_REQUIRE(xin && xout && disjoint(xin, xout, n * sizeof(double)),
+             "temporally blocked variant step: null or overlapping state buffers");
+  const TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N));
+  TbStep k = tb_step_host(p, t, dt);
+  k.fa2 = f_a * f_a;
+  if (var == TB_FIXED_CURVATURE)
+    hipLaunchKernelGGL(k_rk4_tb_var<TB_FIXED_CURVATURE>, dim3(g.total), dim3(1024), 0, st, g, k, xin, xout, psi);
+  else
+    hipLaunchKernelGGL(k_rk4_tb_var<TB_SQRT_POTENTIAL>, dim3(g.total), dim3(1024), 0, st, g, k, xin, xout, psi);
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
+cf_status tb_run_variant(int var, const cf_kg_params *p, double f_a, const double *psi, double *x, double t0,
+                         int64_t k0, int64_t n_steps, double dt, int chunk, void *work, hipStream_t st) {
+  CF_TRY(tb_check(p));
+  CF_REQUIRE(n_steps >= 0, "temporally blocked variant step: negative step count");
+  const size_t bytes = 2 * (size_t)p->N * p->N * p->N * sizeof(double);
+  CF_REQUIRE(x && work && disjoint(x, work, bytes), "temporally blocked variant step: null or overlapping buffer");
+  double *bufs[2] = {x, (double *)work};
+  for (int64_t s = 0; s < n_steps; ++s) {
+    const double t = t0 + static_cast<double>(k0 + s) * dt;
+    CF_TRY(tb_step_variant(var, p, f_a, psi, bufs[s & 1], bufs[(s + 1) & 1], t, dt, chunk, st));
+  }
+  if (n_steps & 1) CF_HIP(hipMemcpyAsync(x, work, bytes, hipMemcpyDeviceToDevice, st));
   return CF_OK;
 }
 
@@ -93,7 +156,8 @@ size_t cf_rk4_tb_work_bytes(int64_t N, int precision) {
 cf_status cf_rk4_run_tb_f64(const cf_kg_params *p, double *x, double t0, int64_t k0, int64_t n_steps, double dt,
                             int chunk, void *work, cf_stream stream) {
   CF_TRY(tb_check(p));
-  CF_REQUIRE(x && work && (void *)x != work, "cf_rk4_run_tb_f64: null or aliased buffer");
+  CF_REQUIRE(x && work && disjoint(x, work, 2 * (size_t)p->N * p->N * p->N * sizeof(double)),
+             "cf_rk4_run_tb_f64: null or overlapping buffer");
   CF_REQUIRE(n_steps >= 0, "cf_rk4_run_tb_f64: negative step count");
   const TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N));
   double *bufs[2] = {x, (double *)work};
@@ -112,10 +176,13 @@ cf_status cf_rk4_run_tb_f64(const cf_kg_params *p, double *x, double t0, int64_t
 cf_status cf_rk4_run_tb_f32c(const cf_kg_params *p, float *hi, float *lo, double t0, int64_t k0, int64_t n_steps,
                              double dt, int chunk, void *work, cf_stream stream) {
   CF_TRY(tb_check(p));
-  CF_REQUIRE(hi && lo && work && hi != lo, "cf_rk4_run_tb_f32c: null or aliased buffer");
+  const size_t n = 2 * (size_t)p->N * p->N * p->N;
+  CF_REQUIRE(hi && lo && work && disjoint(hi, lo, n * sizeof(float)) &&
+                 disjoint(hi, n * sizeof(float), work, 2 * n * sizeof(float)) &&
+                 disjoint(lo, n * sizeof(float), work, 2 * n * sizeof(float)),
+             "cf_rk4_run_tb_f32c: null or overlapping buffer");
   CF_REQUIRE(n_steps >= 0, "cf_rk4_run_tb_f32c: negative step count");
   const TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N));
-  const size_t n = 2 * (size_t)p->N * p->N * p->N;
   float *his[2] = {hi, (float *)work};
   float *los[2] = {lo, (float *)work + n};
   for (int64_t s = 0; s < n_steps; ++s) {
@@ -128,6 +195,28 @@ cf_status cf_rk4_run_tb_f32c(const cf_kg_params *p, float *hi, float *lo, double
     CF_HIP(hipMemcpyAsync(lo, los[1], n * sizeof(float), hipMemcpyDeviceToDevice, S(stream)));
   }
   return CF_OK;
+}
+
+cf_status cf_rk4_step_tb_f64(const cf_kg_params *p, const double *x_in, double *x_out, double t, double dt,
+                             int chunk, cf_stream stream) {
+  CF_TRY(tb_check(p));
+  const size_t n = 2 * (size_t)p->N * p->N * p->N;
+  CF_REQUIRE(x_in && x_out && disjoint(x_in, x_out, n * sizeof(double)),
+             "cf_rk4_step_tb_f64: null or overlapping state buffers");
+  const TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N));
+  return tb_launch<double, false>(g, tb_step_host(p, t, dt), x_in, nullptr, x_out, nullptr, S(stream));
+}
+
+cf_status cf_rk4_step_tb_f32c(const cf_kg_params *p, const float *hi_in, const float *lo_in, float *hi_out,
+                              float *lo_out, double t, double dt, int chunk, cf_stream stream) {
+  CF_TRY(tb_check(p));
+  const size_t b = 2 * (size_t)p->N * p->N * p->N * sizeof(float);
+  CF_REQUIRE(hi_in && lo_in && hi_out && lo_out, "cf_rk4_step_tb_f32c: null buffer");
+  CF_REQUIRE(disjoint(hi_in, lo_in, b) && disjoint(hi_out, lo_out, b) && disjoint(hi_in, hi_out, b) &&
+                 disjoint(hi_in, lo_out, b) && disjoint(lo_in, hi_out, b) && disjoint(lo_in, lo_out, b),
+             "cf_rk4_step_tb_f32c: overlapping buffers");
+  const TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N));
+  return tb_launch<float, true>(g, tb_step_host(p, t, dt), hi_in, lo_in, hi_out, lo_out, S(stream));
 }
 
 }  // extern "C"

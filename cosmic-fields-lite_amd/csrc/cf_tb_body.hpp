@@ -46,7 +46,14 @@ constexpr int TBH = 4;   // halo: one site per stencil stage
 struct TbStep {
   double c_h[4], inv[4];
   double c_m, lam, dh, dt, dt6, dt3;
+  double m, fa2;  // equation variants: mass, f_a^2
 };
+
+// Equation of the step body: the Klein-Gordon family (+ lambda phi^3), or the
+// fixed-curvature / sqrt-potential variants in the operation order of their CUDA
+// kernels (Fixed_Curvature_FRW_equation_kernel, equations_cuda.cu:114-137;
+// Sqrt_Potential_FRW_equation_kernel, :83-106), as in csrc/cf_variants.hip.
+enum : int { TB_KG = 0, TB_FIXED_CURVATURE = 1, TB_SQRT_POTENTIAL = 2 };
 
 // One work unit: lattice size, output tile origin and edge, plane chunk.
 struct TbUnit {
@@ -62,6 +69,21 @@ __device__ __forceinline__ double tb_rhs(double phi, double pa1, double pam, dou
   if (has_lam) v = v - ((lam * phi) * phi) * phi;
   double r = v + inv * ((((-6.0 * phi + pa1) + pam) + pb1) + pbm);
   return r + inv * (pc1 + pcm);
+}
+
+template <int VAR, bool LAM>
+__device__ __forceinline__ double tb_force(double phi, double pa1, double pam, double pb1, double pbm, double pc1,
+                                           double pcm, double pi, double c_h, double inv, const TbStep &k, double ps) {
+  if constexpr (VAR == TB_KG) {
+    return tb_rhs(phi, pa1, pam, pb1, pbm, pc1, pcm, pi, c_h, k.c_m, k.lam, inv, LAM);
+  } else if constexpr (VAR == TB_FIXED_CURVATURE) {
+    const double e2 = exp(2.0 * ps), e4 = exp(4.0 * ps);
+    const double s7 = (((((-6.0 * phi + pa1) + pam) + pb1) + pbm) + pc1) + pcm;
+    return (c_h * pi - ((e2 * k.m) * k.m) * phi) + (e4 * inv) * s7;
+  } else {
+    const double s7 = (((((-6.0 * phi + pa1) + pam) + pb1) + pbm) + pc1) + pcm;
+    return (c_h * pi - (k.c_m * phi) / sqrt(1.0 + (phi * phi) / k.fa2)) + inv * s7;
+  }
 }
 
 // float rounding of the compensated mode (identity for FP64)
@@ -136,10 +158,11 @@ struct TbPending {
 template <class T>
 using TbLds = T[2][4][TBR][TBR];
 
-template <class T, bool F32C, bool LAM, bool DEFER = true, bool LANE_NBR = true>
+template <class T, bool F32C, bool LAM, bool DEFER = true, bool LANE_NBR = true, int VAR = TB_KG>
 __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const TbStep &k, const T *__restrict__ xin,
                                         const float *__restrict__ lin, T *__restrict__ xout,
-                                        float *__restrict__ lout) {
+                                        float *__restrict__ lout, const double *__restrict__ psi = nullptr) {
+  static_assert(VAR == TB_KG || !F32C, "the equation variants are FP64");
   const int tid = threadIdx.x;
   const int r = tid >> 5, cc = tid & 31;
   static_assert(!F32C || sizeof(T) == sizeof(float), "the compensated mode stores float");
@@ -201,8 +224,12 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
   // (loads of it return 0), so no branch surrounds a memory operation.
   int64_t off_ld = wrap_plane(u.a0 - TBH);
   Hv s0f = (Hv)xin[off_ld + col], s0p = (Hv)xin[N3 + off_ld + col];
+  // fixed curvature: Psi rides with x (same slots, same planes)
+  double s0q = 0.0, s1q = 0.0, ps0 = 0.0, ps1 = 0.0, ps2 = 0.0, ps3 = 0.0, ps4 = 0.0;
+  if constexpr (VAR == TB_FIXED_CURVATURE) s0q = psi[off_ld + col];
   off_ld = next_plane(off_ld);
   Hv s1f = (Hv)xin[off_ld + col], s1p = (Hv)xin[N3 + off_ld + col];
+  if constexpr (VAR == TB_FIXED_CURVATURE) s1q = psi[off_ld + col];
   int64_t off_st = wrap_plane(u.a0 - TBH - 4);  // plane P-4
   TbPending<T> q0{}, q1{};
   q0.off = q1.off = TB_OOB;
@@ -230,10 +257,11 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     }
   };
 
-  auto iteration = [&](const int j, Hv &slf, Hv &slp, float &llf, float &llp, T(*buf)[TBR][TBR],
+  auto iteration = [&](const int j, Hv &slf, Hv &slp, double &slq, float &llf, float &llp, T(*buf)[TBR][TBR],
                        TbPending<T> &q) {
     xf0 = slf;
     xp0 = slp;
+    if constexpr (VAR == TB_FIXED_CURVATURE) ps0 = slq;
     double lo_f = 0.0, lo_p = 0.0;
     if constexpr (F32C) {
       lo_f = (double)llf;
@@ -249,6 +277,7 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     off_ld = next_plane(off_ld);  // plane P+2
     slf = (Hv)xin[off_ld + col];
     slp = (Hv)xin[N3 + off_ld + col];
+    if constexpr (VAR == TB_FIXED_CURVATURE) slq = psi[off_ld + col];
 
     buf[0][r][cc] = (T)xf1;
     buf[1][r][cc] = (T)y1f1;
@@ -258,9 +287,9 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
 
     // ---- stage 1 at P-1: k1 = f(x)
     double kf = (double)xp1;
-    double kp = tb_rhs((double)xf1, (double)xf0, (double)xf2, (double)buf[0][rp][cc], (double)buf[0][rm][cc],
-                       cnext(xf1, buf[0]), cprev(xf1, buf[0]), (double)xp1, k.c_h[0], k.c_m, k.lam,
-                       k.inv[0], LAM);
+    double kp = tb_force<VAR, LAM>((double)xf1, (double)xf0, (double)xf2, (double)buf[0][rp][cc],
+                                  (double)buf[0][rm][cc], cnext(xf1, buf[0]), cprev(xf1, buf[0]),
+                                  (double)xp1, k.c_h[0], k.inv[0], k, ps1);
     const Hv y1f0 = (Hv)rnd<F32C>((double)xf1 + k.dh * kf);
     const Hv y1p0 = (Hv)rnd<F32C>((double)xp1 + k.dh * kp);
     Hv af0, ap0;
@@ -274,9 +303,9 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
 
     // ---- stage 2 at P-2: k2 = f(y1)
     kf = (double)y1p1;
-    kp = tb_rhs((double)y1f1, (double)y1f0, (double)y1f2, (double)buf[1][rp][cc], (double)buf[1][rm][cc],
-                cnext(y1f1, buf[1]), cprev(y1f1, buf[1]), (double)y1p1, k.c_h[1], k.c_m, k.lam, k.inv[1],
-                LAM);
+    kp = tb_force<VAR, LAM>((double)y1f1, (double)y1f0, (double)y1f2, (double)buf[1][rp][cc],
+                                  (double)buf[1][rm][cc], cnext(y1f1, buf[1]), cprev(y1f1, buf[1]),
+                                  (double)y1p1, k.c_h[1], k.inv[1], k, ps2);
     const Hv y2f0 = (Hv)rnd<F32C>((double)xf2 + k.dh * kf);
     const Hv y2p0 = (Hv)rnd<F32C>((double)xp2 + k.dh * kp);
     af1 = (Hv)((double)af1 + k.dt3 * kf);
@@ -284,9 +313,9 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
 
     // ---- stage 3 at P-3: k3 = f(y2)
     kf = (double)y2p1;
-    kp = tb_rhs((double)y2f1, (double)y2f0, (double)y2f2, (double)buf[2][rp][cc], (double)buf[2][rm][cc],
-                cnext(y2f1, buf[2]), cprev(y2f1, buf[2]), (double)y2p1, k.c_h[2], k.c_m, k.lam, k.inv[2],
-                LAM);
+    kp = tb_force<VAR, LAM>((double)y2f1, (double)y2f0, (double)y2f2, (double)buf[2][rp][cc],
+                                  (double)buf[2][rm][cc], cnext(y2f1, buf[2]), cprev(y2f1, buf[2]),
+                                  (double)y2p1, k.c_h[2], k.inv[2], k, ps3);
     const Hv y3f0 = (Hv)rnd<F32C>((double)xf3 + k.dt * kf);
     const Hv y3p0 = (Hv)rnd<F32C>((double)xp3 + k.dt * kp);
     af2 = (Hv)((double)af2 + k.dt3 * kf);
@@ -294,9 +323,9 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
 
     // ---- stage 4 at P-4: k4 = f(y3); new state
     kf = (double)y3p1;
-    kp = tb_rhs((double)y3f1, (double)y3f0, (double)y3f2, (double)buf[3][rp][cc], (double)buf[3][rm][cc],
-                cnext(y3f1, buf[3]), cprev(y3f1, buf[3]), (double)y3p1, k.c_h[3], k.c_m, k.lam, k.inv[3],
-                LAM);
+    kp = tb_force<VAR, LAM>((double)y3f1, (double)y3f0, (double)y3f2, (double)buf[3][rp][cc],
+                                  (double)buf[3][rm][cc], cnext(y3f1, buf[3]), cprev(y3f1, buf[3]),
+                                  (double)y3p1, k.c_h[3], k.inv[3], k, ps4);
     if constexpr (F32C) {
       const double s0 = ((double)xf4 + lo_f) + ((double)af3 + k.dt6 * kf);
       const double s1 = ((double)xp4 + lo_p) + ((double)ap3 + k.dt6 * kp);
@@ -316,6 +345,7 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
 
     // ---- advance the histories by one plane
     xf4 = xf3; xf3 = xf2; xf2 = xf1; xf1 = xf0;
+    ps4 = ps3; ps3 = ps2; ps2 = ps1; ps1 = ps0;
     xp4 = xp3; xp3 = xp2; xp2 = xp1; xp1 = xp0;
     y1f2 = y1f1; y1f1 = y1f0; y1p1 = y1p0;
     y2f2 = y2f1; y2f1 = y2f0; y2p1 = y2p0;
@@ -325,10 +355,10 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
   };
   int j = 0;
   for (; j + 1 < iters; j += 2) {
-    iteration(j, s0f, s0p, l0f, l0p, lds[0], q0);
-    iteration(j + 1, s1f, s1p, l1f, l1p, lds[1], q1);
+    iteration(j, s0f, s0p, s0q, l0f, l0p, lds[0], q0);
+    iteration(j + 1, s1f, s1p, s1q, l1f, l1p, lds[1], q1);
   }
-  if (j < iters) iteration(j, s0f, s0p, l0f, l0p, lds[0], q0);
+  if (j < iters) iteration(j, s0f, s0p, s0q, l0f, l0p, lds[0], q0);
   if constexpr (DEFER) {
     flush(q0);
     flush(q1);
@@ -352,6 +382,8 @@ inline TbStep tb_step_host(const cf_kg_params *p, double t, double dt) {
   }
   k.c_m = p->m * p->m;
   k.lam = p->lambda;
+  k.m = p->m;
+  k.fa2 = 0.0;
   k.dh = dh;
   k.dt = dt;
   k.dt6 = dt / 6.0;

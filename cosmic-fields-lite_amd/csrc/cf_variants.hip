@@ -33,6 +33,7 @@
 #include <cmath>
 
 #include "cf_march.hpp"
+#include "cf_tb_run.hpp"
 
 namespace cf {
 
@@ -541,6 +542,27 @@ cf_status cf_variant_rk4_run(cf_variant v, double *x, double t0, int64_t k0, int
     CF_TRY(cf_variant_rk4_step(v, x, t, dt, work, stream));
   }
   return CF_OK;
+}
+
+cf_status cf_variant_rk4_run_tb(cf_variant v, double *x, double t0, int64_t k0, int64_t n_steps, double dt, int chunk,
+                                void *work, cf_stream stream) {
+  CF_REQUIRE(v, "cf_variant_rk4_run_tb: null handle");
+  if (v->kind != V_FIXED && v->kind != V_SQRT) {
+    set_error("cf_variant_rk4_run_tb: the comoving variants rebuild Psi at every stage time; use cf_variant_rk4_run");
+    return CF_E_UNSUPPORTED;
+  }
+  // TB_FIXED_CURVATURE = 1 = CF_VAR_FIXED_CURVATURE, TB_SQRT_POTENTIAL = 2 = CF_VAR_SQRT_POTENTIAL
+  return tb_run_variant(v->kind, &v->p, v->f_a, v->psi_fixed, x, t0, k0, n_steps, dt, chunk, work, S(stream));
+}
+
+cf_status cf_variant_rk4_step_tb(cf_variant v, const double *x_in, double *x_out, double t, double dt, int chunk,
+                                 cf_stream stream) {
+  CF_REQUIRE(v, "cf_variant_rk4_step_tb: null handle");
+  if (v->kind != V_FIXED && v->kind != V_SQRT) {
+    set_error("cf_variant_rk4_step_tb: the comoving variants rebuild Psi at every stage time; use cf_variant_rk4_step");
+    return CF_E_UNSUPPORTED;
+  }
+  return tb_step_variant(v->kind, &v->p, v->f_a, v->psi_fixed, x_in, x_out, t, dt, chunk, S(stream));
 }
 
 cf_status cf_variant_psi(cf_variant v, double t, double *psi, double *dpsi, cf_stream stream) {

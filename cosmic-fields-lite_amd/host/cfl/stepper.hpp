@@ -2,15 +2,20 @@
 //
 // hip_runge_kutta4 has the interface of an odeint explicit stepper (the shape
 // of the reference's own custom stepper, src/midpoint.hpp:25-101): do_step(
-// system, x, t, dt). If the system provides rk4_step(x, t, dt) -- every HIP
-// equation in equations_hip.hpp does -- the step runs as four fused stage
-// kernels; otherwise it falls back to the classic sys() + scale_sum sequence.
-// Both give the same bits. Works with cfl::odeint_lite::integrate_const and,
-// when Boost.odeint is present, with boost::numeric::odeint::integrate_const.
+// system, x, t, dt). If the system provides rk4_step_into(x, out, t, dt) (the
+// Klein-Gordon, lambda, fixed-curvature and sqrt-potential equations) the step
+// is one temporally blocked kernel from x into a spare state, and then x and the
+// spare trade buffers (x.swap), so x's storage alternates between two
+// allocations; else, if it provides rk4_step(x, t, dt) (the comoving
+// equations), four fused stage kernels in place; otherwise the classic sys() +
+// scale_sum sequence. All three give the same bits. Works with
+// cfl::odeint_lite::integrate_const and, when Boost.odeint is present, with
+// boost::numeric::odeint::integrate_const.
 //
 // hip_runge_kutta4_f32c steps a compensated_state: fields stored as float
 // hi + lo, stage inputs and the RK4 increment stored as float, all arithmetic
-// in double (cf_rk4_step_f32c).
+// in double (the temporally blocked cf_rk4_step_tb_f32c, same bits as
+// cf_rk4_step_f32c), with the same buffer swap.
 #pragma once
 
 #if __has_include(<boost/numeric/odeint/stepper/stepper_categories.hpp>)
@@ -40,13 +45,19 @@ class hip_runge_kutta4 {
 
   template <class System>
   void do_step(System system, State &x, double t, double dt) {
-    if constexpr (requires { system.rk4_step(x, t, dt); })
+    if constexpr (requires { system.rk4_step_into(x, x, t, dt); }) {
+      // one kernel into the spare buffer, then the two buffers trade places
+      system.rk4_step_into(x, *next_, t, dt);
+      x.swap(*next_);
+    } else if constexpr (requires { system.rk4_step(x, t, dt); }) {
       system.rk4_step(x, t, dt);
-    else
+    } else {
       fallback_->do_step(system, x, t, dt);
+    }
   }
 
  private:
+  std::shared_ptr<State> next_ = std::make_shared<State>();
   std::shared_ptr<odeint_lite::runge_kutta4_classic<State>> fallback_ =
       std::make_shared<odeint_lite::runge_kutta4_classic<State>>();
 };
@@ -84,14 +95,20 @@ class hip_runge_kutta4_f32c {
   template <class System>
   void do_step(System system, compensated_state &x, double t, double dt) {
     const cf_kg_params p = system.params();
-    const std::size_t wb = cf_rk4_work_bytes(p.N, CF_PREC_F32C);
-    if (work_->size() != wb) work_->resize(wb);
-    check(cf_rk4_step_f32c(&p, x.hi.data(), x.lo.data(), t, dt, work_->data(), default_stream()),
+    compensated_state &o = *next_;
+    if (o.size() != x.size()) {
+      o.hi.resize(x.size());
+      o.lo.resize(x.size());
+    }
+    // the temporally blocked kernel (same bits as cf_rk4_step_f32c), then a swap
+    check(cf_rk4_step_tb_f32c(&p, x.hi.data(), x.lo.data(), o.hi.data(), o.lo.data(), t, dt, 0, default_stream()),
           "hip_runge_kutta4_f32c");
+    x.hi.swap(o.hi);
+    x.lo.swap(o.lo);
   }
 
  private:
-  std::shared_ptr<hip_vector<unsigned char>> work_ = std::make_shared<hip_vector<unsigned char>>();
+  std::shared_ptr<compensated_state> next_ = std::make_shared<compensated_state>();
 };
 
 }  // namespace cfl

@@ -134,6 +134,13 @@ cf_status cf_rk4_run_tb_f64(const cf_kg_params *p, double *x_dev, double t0, int
                             double dt, int chunk, void *work_dev, cf_stream stream);
 cf_status cf_rk4_run_tb_f32c(const cf_kg_params *p, float *hi_dev, float *lo_dev, double t0, int64_t k0,
                              int64_t n_steps, double dt, int chunk, void *work_dev, cf_stream stream);
+/* One temporally blocked step out of place (x_out = step(x_in); the buffers must not
+ * overlap; no work buffer): the form a stepper that swaps its state buffers uses. */
+cf_status cf_rk4_step_tb_f64(const cf_kg_params *p, const double *x_in_dev, double *x_out_dev, double t, double dt,
+                             int chunk, cf_stream stream);
+cf_status cf_rk4_step_tb_f32c(const cf_kg_params *p, const float *hi_in_dev, const float *lo_in_dev,
+                              float *hi_out_dev, float *lo_out_dev, double t, double dt, int chunk,
+                              cf_stream stream);
 
 /* Split a double state into hi/lo floats and back (x = hi + lo). */
 cf_status cf_split_f64(const double *x_dev, float *hi_dev, float *lo_dev, int64_t n, cf_stream stream);
@@ -238,6 +245,14 @@ cf_status cf_variant_rhs(cf_variant v, const double *x_dev, double *dxdt_dev, do
 cf_status cf_variant_rk4_step(cf_variant v, double *x_dev, double t, double dt, void *work_dev, cf_stream stream);
 cf_status cf_variant_rk4_run(cf_variant v, double *x_dev, double t0, int64_t k0, int64_t n_steps, double dt,
                              void *work_dev, cf_stream stream);
+/* The same steps temporally blocked (one kernel per step, as cf_rk4_run_tb_f64) for
+ * CF_VAR_FIXED_CURVATURE and CF_VAR_SQRT_POTENTIAL; same bits as cf_variant_rk4_run.
+ * work_dev: cf_rk4_tb_work_bytes(N, CF_PREC_F64). The comoving variants rebuild Psi at
+ * every stage time and return CF_E_UNSUPPORTED. */
+cf_status cf_variant_rk4_run_tb(cf_variant v, double *x_dev, double t0, int64_t k0, int64_t n_steps, double dt,
+                                int chunk, void *work_dev, cf_stream stream);
+cf_status cf_variant_rk4_step_tb(cf_variant v, const double *x_in_dev, double *x_out_dev, double t, double dt,
+                                 int chunk, cf_stream stream);
 /* Equation::compute_energy_density (equations_cuda.cu:422-473, equations.cpp:262-322). */
 cf_status cf_variant_energy_density(cf_variant v, const double *x_dev, double *rho_dev, double t,
                                     cf_stream stream);

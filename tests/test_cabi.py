@@ -102,6 +102,13 @@ def test_bad_arguments_rejected_before_device_work(lib):
     assert L.cf_variant_create(C.byref(p), lib.CF_VAR_APPROX_COMOVING_CURVATURE, 1.0, 6, fake, C.byref(h),
                                None) == lib.CF_E_INVALID  # M must divide N
     assert h.value is None
+    assert L.cf_variant_rk4_run_tb(None, fake, 1.0, 0, 1, 0.1, 0, fake, None) == lib.CF_E_INVALID  # null handle
+    assert L.cf_variant_rk4_step_tb(None, fake, fake, 1.0, 0.1, 0, None) == lib.CF_E_INVALID
+    x0 = C.c_void_p(0x100000)
+    assert L.cf_rk4_step_tb_f64(C.byref(p), x0, C.c_void_p(0x100000 + 8), 1.0, 0.1, 0, None) == \
+        lib.CF_E_INVALID  # overlapping in/out
+    assert L.cf_rk4_step_tb_f32c(C.byref(p), x0, C.c_void_p(0x900000), x0, C.c_void_p(0xA00000), 1.0, 0.1, 0,
+                                 None) == lib.CF_E_INVALID
     pw = lib.KGParams(16, 16.0, 1.0, 0.0, lib.Cosmology(1.0, 0.5, 3.0, 1.0))  # t1 != 1/(2 H1)
     assert L.cf_wkb_create(C.byref(pw), fake, 3.0, C.byref(h), None) == lib.CF_E_INVALID
     assert b"radiation" in L.cf_last_error()

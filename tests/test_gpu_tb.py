@@ -66,3 +66,56 @@ def test_tb_equals_stage_kernels_large(cfl, N):
     cfl.rk4_run_tb_f32c(p, h2, l2, 1.0, 0, 2, 0.05, cfl.alloc_tb_work(N, cfl.CF_PREC_F32C))
     torch.cuda.synchronize()
     assert torch.equal(hi, h2) and torch.equal(lo, l2)
+
+
+@pytest.mark.parametrize("N,lam", [(16, 0.1), (40, 0.0)])
+def test_tb_single_step_out_of_place(cfl, O, N, lam):
+    """cf_rk4_step_tb_f64 / _f32c (input left untouched, result in the second
+    buffer) give the in-place step's bits; chained by swapping buffers they give
+    the oracle's three steps."""
+    print(f"case tb single step N={N} lam={lam}", flush=True)
+    L = float(N)
+    eq = O.Equation(N, L, 1.0, lam, O.Cosmo.radiation(**FRW))
+    p = cfl.kg_params(N, L, 1.0, lam, cfl.radiation(**FRW))
+    x = rand_state(N, 23, amp=0.5)
+    a, b = dev(x), torch.empty(x.size, dtype=torch.float64, device="cuda")
+    for k in range(3):
+        cfl.rk4_step_tb_f64(p, a, b, 1.0 + k * 0.05, 0.05)
+        if k == 0:
+            assert_bitwise(host(a), x, "input untouched")
+        a, b = b, a
+        eq.rk4_step(x, 1.0 + k * 0.05, 0.05)
+    assert_bitwise(host(a), x, "three swapped steps")
+    y = rand_state(N, 24, amp=0.5)
+    hi = y.astype(np.float32)
+    lo = (y - hi.astype(np.float64)).astype(np.float32)
+    hd, ld = dev(hi.copy()), dev(lo.copy())
+    ho, lo_o = torch.empty_like(hd), torch.empty_like(ld)
+    cfl.rk4_step_tb_f32c(p, hd, ld, ho, lo_o, 1.0, 0.05)
+    eq.rk4_step_f32c(hi, lo, 1.0, 0.05)
+    assert_bitwise(host(ho), hi, "f32c hi")
+    assert_bitwise(host(lo_o), lo, "f32c lo")
+
+
+def test_tb_overlap_checks(cfl):
+    """Overlapping buffers are refused; adjacent ones of unequal length (hi, then
+    the 2x larger work buffer, then lo, in one allocation) are accepted."""
+    N = 16
+    p = cfl.kg_params(N, 16.0, 1.0, 0.0, cfl.radiation(**FRW))
+    x = torch.zeros(4 * N ** 3, dtype=torch.float64, device="cuda")
+    with pytest.raises(cfl.CfError):
+        cfl.rk4_step_tb_f64(p, x, x[N ** 3:], 1.0, 0.05)  # half overlap
+    with pytest.raises(cfl.CfError):
+        cfl.rk4_run_tb_f64(p, x, 1.0, 0, 1, 0.05, x[N ** 3:])
+    n = 2 * N ** 3
+    y = rand_state(N, 25, amp=0.5)
+    hi = y.astype(np.float32)
+    lo = (y - hi.astype(np.float64)).astype(np.float32)
+    big = torch.empty(4 * n, dtype=torch.float32, device="cuda")
+    big[:n] = dev(hi)
+    big[3 * n:] = dev(lo)
+    cfl.rk4_run_tb_f32c(p, big[:n], big[3 * n:], 1.0, 0, 1, 0.05, big[n:3 * n])
+    h2, l2 = dev(hi), dev(lo)
+    cfl.rk4_run_f32c(p, h2, l2, 1.0, 0, 1, 0.05, cfl.alloc_work(N, cfl.CF_PREC_F32C))
+    torch.cuda.synchronize()
+    assert torch.equal(big[:n], h2) and torch.equal(big[3 * n:], l2)

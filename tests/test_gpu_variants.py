@@ -168,3 +168,38 @@ def test_bad_arguments(cfl):
         cfl.Variant(p, cfl.CF_VAR_APPROX_COMOVING_CURVATURE, R, M=6)  # M must divide N
     with pytest.raises(cfl.CfError):
         cfl.Variant(p, cfl.CF_VAR_SQRT_POTENTIAL, f_a=0.0)
+
+
+@pytest.mark.parametrize("name,kind,N,steps,chunk", [("fixed", 1, 16, 3, 0), ("fixed", 1, 40, 2, 7),
+                                                      ("fixed", 1, 64, 1, 0), ("sqrt", 2, 16, 3, 0),
+                                                      ("sqrt", 2, 40, 2, 5), ("sqrt", 2, 96, 1, 0)])
+def test_rk4_tb_equals_stage_kernels(cfl, O, name, kind, N, steps, chunk):
+    """The temporally blocked variant step gives the four-kernel step's bits (ragged
+    tiles at N=40, a chunk that does not divide N, odd and even step counts); the
+    sqrt potential's also the oracle's."""
+    print(f"case rk4_tb {name} N={N} steps={steps} chunk={chunk}", flush=True)
+    v, ov = setup(cfl, O, kind, N)
+    x0 = rand_state(N, 12, 0.3)
+    xa, xb = dev(x0), dev(x0)
+    t0, dt = 1.0, 0.05
+    v.rk4_run(xa, t0, 2, steps, dt)
+    v.rk4_run_tb(xb, t0, 2, steps, dt, chunk=chunk)
+    assert_bitwise(host(xb), host(xa), f"{name} tb vs stage kernels")
+    xc, xo = dev(x0), torch.empty_like(xa)
+    for s in range(steps):  # the out-of-place single step, buffers swapped
+        v.rk4_step_tb(xc, xo, t0 + float(2 + s) * dt, dt, chunk=chunk)
+        xc, xo = xo, xc
+    assert_bitwise(host(xc), host(xa), f"{name} single tb steps vs stage kernels")
+    if kind == O.SQRT and N <= 40:
+        assert_bitwise(host(xb), ov.rk4_run(x0.copy(), t0, dt, steps, k0=2), f"{name} tb vs oracle")
+    v.close()
+
+
+def test_rk4_tb_comoving_unsupported(cfl, O):
+    v, _ = setup(cfl, O, O.COMOVING, 16)
+    x = dev(rand_state(16, 1, 0.3))
+    with pytest.raises(cfl.CfError):
+        v.rk4_run_tb(x, 1.0, 0, 1, 0.05)
+    with pytest.raises(cfl.CfError):
+        v.rk4_step_tb(x, torch.empty_like(x), 1.0, 0.05)
+    v.close()
