@@ -21,6 +21,7 @@ host on a bounded sample.
     python bench.py --workload precision --steps 1000     # run A vs run B errors
     python bench.py --workload ensemble --steps 500       # 96 uneven-N realisations
     python bench.py --workload adjoint                    # loss + gradient, N=128, 1024 steps
+    python bench.py --workload variants --steps 10        # fixed-curvature / sqrt-potential steps
 """
 import argparse
 import json
@@ -78,7 +79,8 @@ def parse():
                     help="headline: kernels to time (tb_f32c, tb_f64: one temporally blocked kernel per step; "
                          "f32c, f64: four fused stage kernels per step)")
     ap.add_argument("--save-dir", default="", help="ensemble: where to save the averaged density spectra (.npy)")
-    ap.add_argument("--workload", choices=("headline", "ensemble", "precision", "adjoint"), default="headline",
+    ap.add_argument("--workload", choices=("headline", "ensemble", "precision", "adjoint", "variants"),
+                    default="headline",
                     help="headline: N=512 FRW, FP32-comp vs FP64; ensemble: 96 realisations of N=64/96/128; "
                          "precision: run A (FP32-comp) vs run B (FP64) errors every 100 steps; "
                          "adjoint: loss + gradient through 1024 steps at N=128")
@@ -318,8 +320,58 @@ def adjoint_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def variants_workload(args):
+    """FP64 RK4 steps of the fixed-curvature (Psi: a Gaussian random field, rms 0.1)
+    and sqrt-potential (f_a = 0.7) equations at N=512, L=512, m=1.1, radiation FRW,
+    dt=0.01, from the headline's initial state: the four fused stage kernels per step
+    against the temporally blocked kernel (same bits), HIP events on the launch stream."""
+    import torch
+    import cfl_amd as cfl
+    N = args.n
+    p = cfl.kg_params(N, float(N), 1.1, 0.0, cfl.radiation(a1=1.0, H1=0.5, t1=1.0))
+    plan = cfl.SpectrumPlan(N)
+    x0 = plan.init_grf_state(2, cfl.CF_SPEC_GAUSS, (0.25,), float(N), 1.0)
+    psi = plan.init_grf_state(3, cfl.CF_SPEC_GAUSS, (0.25,), float(N), 0.1)[: N ** 3].clone()
+    del plan
+    K, W = args.steps, max(1, args.warmup)
+    out = {}
+    for name, kind, field in (("fixed_curvature", cfl.CF_VAR_FIXED_CURVATURE, psi),
+                              ("sqrt_potential", cfl.CF_VAR_SQRT_POTENTIAL, None)):
+        v = cfl.Variant(p, kind, field, f_a=0.7)
+        for sched in ("stage_kernels", "tb"):
+            x = x0.clone()
+            run = v.rk4_run if sched == "stage_kernels" else v.rk4_run_tb
+            run(x, 1.0, 0, W, 0.01)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            run(x, 1.0, W, K, 0.01)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / K
+            out[f"{name}_{sched}"] = {"ms_per_step": ms, "gsite_steps_per_s": N ** 3 / ms / 1e6}
+            del x
+        v.close()
+    line = {
+        "metric": "lattice site-steps per second, equation variants", "value": out["fixed_curvature_tb"][
+            "gsite_steps_per_s"], "unit": "GSite*step/s (fixed curvature, temporally blocked)",
+        "higher_is_better": True, "n_gpus": 1, "steps": K, "warmup": W, "dtype": "f64",
+        "data": "synthetic (Gaussian random fields, seeds 2 and 3)",
+        "config": {"workload": f"fixed-curvature and sqrt-potential FRW equations N={N}^3 L={N} m=1.1 f_a=0.7 "
+                               "dt=0.01 RK4"},
+        "paths": out,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
+    if args.workload == "variants":
+        if args.gpus != 1:
+            print("bench.py: this build runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        variants_workload(args)
+        return
     if args.workload == "adjoint":
         if args.gpus != 1:
             print("bench.py: this build runs on one GPU only", file=sys.stderr)

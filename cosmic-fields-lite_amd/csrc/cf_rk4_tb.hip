@@ -67,7 +67,7 @@ __global__ __launch_bounds__(1024) void k_rk4_tb_var(TbGeom g, TbStep k, const d
   u.c0 = (int64_t)tc * g.T;
   u.a0 = (int64_t)ac * g.A;
   u.a1 = (u.a0 + g.A < g.N) ? u.a0 + g.A : g.N;
-  tb_unit<double, false, false, true, true, VAR>(lds, u, k, xin, nullptr, xout, nullptr, psi);
+  tb_unit<double, false, false, true, true, VAR, true>(lds, u, k, xin, nullptr, xout, nullptr, psi);
 }
 
 static TbGeom tb_geom(int64_t N, int A) {

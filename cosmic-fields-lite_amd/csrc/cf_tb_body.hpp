@@ -119,7 +119,9 @@ __device__ __forceinline__ float tb_load_f32(__amdgpu_buffer_rsrc_t r, uint32_t 
 // results are never used, exactly as with clamped LDS indices.
 template <int CTRL>
 __device__ __forceinline__ int tb_dpp(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+  // bound_ctrl: a lane whose source is outside the wavefront reads 0 (no "old"
+  // value to initialise first)
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
 }
 template <int CTRL, class V>
 __device__ __forceinline__ V tb_lane_shift(V v) {
@@ -158,7 +160,7 @@ struct TbPending {
 template <class T>
 using TbLds = T[2][4][TBR][TBR];
 
-template <class T, bool F32C, bool LAM, bool DEFER = true, bool LANE_NBR = true, int VAR = TB_KG>
+template <class T, bool F32C, bool LAM, bool DEFER = true, bool LANE_NBR = true, int VAR = TB_KG, bool RING = false>
 __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const TbStep &k, const T *__restrict__ xin,
                                         const float *__restrict__ lin, T *__restrict__ xout,
                                         float *__restrict__ lout, const double *__restrict__ psi = nullptr) {
@@ -205,47 +207,58 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     return off == N3 ? 0 : off;
   };
 
-  Hv xf0 = 0, xf1 = 0, xf2 = 0, xf3 = 0, xf4 = 0;  // x phi at P, P-1, ..., P-4
-  Hv xp0 = 0, xp1 = 0, xp2 = 0, xp3 = 0, xp4 = 0;  // x pi
-  Hv y1f1 = 0, y1f2 = 0, y1p1 = 0;                  // y1 phi at P-2, P-3 (start of iteration); pi at P-2
-  Hv y2f1 = 0, y2f2 = 0, y2p1 = 0;                  // y2 phi at P-3, P-4; pi at P-3
-  Hv y3f1 = 0, y3f2 = 0, y3p1 = 0;                  // y3 phi at P-4, P-5; pi at P-4
-  Hv af1 = 0, af2 = 0, af3 = 0, ap1 = 0, ap2 = 0, ap3 = 0;  // acc (FP64) / increment D (F32C) at P-2, P-3, P-4
+  // Plane histories are arrays of TBU entries; the value d planes behind the
+  // iteration's phase PH sits at [(PH - d) mod TBU]. Ring XF / XP holds x by plane (P
+  // at [PH], P-d at [PH-d], the prefetched P+1, P+2 at [PH+1], [PH+2]); Y1F.. hold
+  // the stage inputs made at P-1 by the iteration that made them; AF / AP the
+  // partial sum (FP64) or increment D (F32C). The longest-lived value (x at P+2 ..
+  // P-4) spans 7 iterations, hence TBU = 8. Two ways to advance one plane:
+  //  RING: the loop is unrolled by TBU and PH = j mod TBU is a compile-time constant
+  //        in each copy, so advancing renames registers (no moves; larger code);
+  //  shift (default): PH = 0 throughout and every array rotates by one entry after
+  //        each iteration (register moves), loop unrolled by two.
+  // Which is faster depends on the kernel's register pressure and instruction mix:
+  // the curvature variant gains from RING (it would otherwise spill), the plain
+  // Klein-Gordon kernels do not.
+  constexpr int TBU = 8;
+  Hv XF[TBU] = {}, XP[TBU] = {};
+  Hv Y1F[TBU] = {}, Y1P[TBU] = {}, Y2F[TBU] = {}, Y2P[TBU] = {}, Y3F[TBU] = {}, Y3P[TBU] = {};
+  Hv AF[TBU] = {}, AP[TBU] = {};
+  double PS[TBU] = {};  // fixed curvature: Psi rides with x (same ring, same planes)
 
   const int iters = (int)(u.a1 - u.a0) + 8;
-  // Memory pipeline. Two prefetch slots are used alternately: iteration j takes
-  // x at plane P from slot j&1 (loaded two iterations earlier) and refills it
-  // with plane P+2. The new state of iteration j is stored at the start of
-  // iteration j+2. The loop is unrolled by two, so slots and pending stores are
-  // fixed registers and every wait is for an operation issued at least an
-  // iteration earlier (DEFER = false stores at the end of the iteration, for
-  // callers with less register room). Stores go through buffer resources over one plane: lanes
-  // that do not store get an offset past its end, which the hardware drops
-  // (loads of it return 0), so no branch surrounds a memory operation.
+  // Memory pipeline. Iteration j computes with x at plane P (loaded two iterations
+  // earlier) and loads plane P+2. The new state of iteration j is stored at the
+  // start of iteration j+2 (DEFER = false stores at the end of the iteration, for
+  // callers with less register room), so every wait is for an operation issued at
+  // least an iteration earlier. Stores go through buffer resources over one plane:
+  // lanes that do not store get an offset past its end, which the hardware drops
+  // (loads of it return 0), so no branch surrounds a memory operation. With RING
+  // the loop runs a multiple of TBU iterations; those past `iters` store nothing.
   int64_t off_ld = wrap_plane(u.a0 - TBH);
-  Hv s0f = (Hv)xin[off_ld + col], s0p = (Hv)xin[N3 + off_ld + col];
-  // fixed curvature: Psi rides with x (same slots, same planes)
-  double s0q = 0.0, s1q = 0.0, ps0 = 0.0, ps1 = 0.0, ps2 = 0.0, ps3 = 0.0, ps4 = 0.0;
-  if constexpr (VAR == TB_FIXED_CURVATURE) s0q = psi[off_ld + col];
+  XF[0] = (Hv)xin[off_ld + col];
+  XP[0] = (Hv)xin[N3 + off_ld + col];
+  if constexpr (VAR == TB_FIXED_CURVATURE) PS[0] = psi[off_ld + col];
   off_ld = next_plane(off_ld);
-  Hv s1f = (Hv)xin[off_ld + col], s1p = (Hv)xin[N3 + off_ld + col];
-  if constexpr (VAR == TB_FIXED_CURVATURE) s1q = psi[off_ld + col];
+  XF[1] = (Hv)xin[off_ld + col];
+  XP[1] = (Hv)xin[N3 + off_ld + col];
+  if constexpr (VAR == TB_FIXED_CURVATURE) PS[1] = psi[off_ld + col];
   int64_t off_st = wrap_plane(u.a0 - TBH - 4);  // plane P-4
-  TbPending<T> q0{}, q1{};
-  q0.off = q1.off = TB_OOB;
+  TbPending<T> Q[2] = {};
+  Q[0].off = Q[1].off = TB_OOB;
   // F32C: the compensation term of output plane P-4 is prefetched the same way
   // (slot j&1, loaded two iterations ahead; lanes that will not store read 0).
-  float l0f = 0.0f, l0p = 0.0f, l1f = 0.0f, l1p = 0.0f;
+  float LF[2] = {0.0f, 0.0f}, LP[2] = {0.0f, 0.0f};
   int64_t off_lo = off_st;
   auto lo_off = [&](int jj) -> uint32_t {
-    return ((jj >= 8) && out_ok) ? (uint32_t)(col * (int64_t)sizeof(float)) : TB_OOB;
+    return ((jj >= 8) && (jj < iters) && out_ok) ? (uint32_t)(col * (int64_t)sizeof(float)) : TB_OOB;
   };
   if constexpr (F32C) {
-    l0f = tb_load_f32(tb_plane(lin + off_lo, plane_bytes_f), lo_off(0));
-    l0p = tb_load_f32(tb_plane(lin + N3 + off_lo, plane_bytes_f), lo_off(0));
+    LF[0] = tb_load_f32(tb_plane(lin + off_lo, plane_bytes_f), lo_off(0));
+    LP[0] = tb_load_f32(tb_plane(lin + N3 + off_lo, plane_bytes_f), lo_off(0));
     off_lo = next_plane(off_lo);
-    l1f = tb_load_f32(tb_plane(lin + off_lo, plane_bytes_f), lo_off(1));
-    l1p = tb_load_f32(tb_plane(lin + N3 + off_lo, plane_bytes_f), lo_off(1));
+    LF[1] = tb_load_f32(tb_plane(lin + off_lo, plane_bytes_f), lo_off(1));
+    LP[1] = tb_load_f32(tb_plane(lin + N3 + off_lo, plane_bytes_f), lo_off(1));
   }
 
   auto flush = [&](const TbPending<T> &q) {
@@ -257,111 +270,142 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     }
   };
 
-  auto iteration = [&](const int j, Hv &slf, Hv &slp, double &slq, float &llf, float &llp, T(*buf)[TBR][TBR],
-                       TbPending<T> &q) {
-    xf0 = slf;
-    xp0 = slp;
-    if constexpr (VAR == TB_FIXED_CURVATURE) ps0 = slq;
+  auto iteration = [&](auto phase, auto slot, const int j) {
+    constexpr int PH = decltype(phase)::value;
+    constexpr int I0 = PH % TBU, I1 = (PH + TBU - 1) % TBU, I2 = (PH + TBU - 2) % TBU;
+    constexpr int I3 = (PH + TBU - 3) % TBU, I4 = (PH + TBU - 4) % TBU, IN = (PH + 2) % TBU;
+    constexpr int S = decltype(slot)::value;  // j & 1: LDS set, pending store, lo slot
+    // RING: keep the scheduler within one iteration (interleaving copies of the
+    // unrolled body lengthens live ranges past the register budget of 1024-thread
+    // blocks)
+    if constexpr (RING) __builtin_amdgcn_sched_barrier(0);
+    T(*buf)[TBR][TBR] = lds[S];
+    TbPending<T> &q = Q[S];
     double lo_f = 0.0, lo_p = 0.0;
     if constexpr (F32C) {
-      lo_f = (double)llf;
-      lo_p = (double)llp;
+      lo_f = (double)LF[S];
+      lo_p = (double)LP[S];
     }
     if constexpr (DEFER) flush(q);  // the new state of iteration j-2
-    const uint32_t st_off = ((j >= 8) && out_ok) ? (uint32_t)(col * (int64_t)sizeof(T)) : TB_OOB;
+    const uint32_t st_off = ((j >= 8) && (j < iters) && out_ok) ? (uint32_t)(col * (int64_t)sizeof(T)) : TB_OOB;
     if constexpr (F32C) {
       off_lo = next_plane(off_lo);  // plane P-2, the output plane of iteration j+2
-      llf = tb_load_f32(tb_plane(lin + off_lo, plane_bytes_f), lo_off(j + 2));
-      llp = tb_load_f32(tb_plane(lin + N3 + off_lo, plane_bytes_f), lo_off(j + 2));
+      LF[S] = tb_load_f32(tb_plane(lin + off_lo, plane_bytes_f), lo_off(j + 2));
+      LP[S] = tb_load_f32(tb_plane(lin + N3 + off_lo, plane_bytes_f), lo_off(j + 2));
     }
     off_ld = next_plane(off_ld);  // plane P+2
-    slf = (Hv)xin[off_ld + col];
-    slp = (Hv)xin[N3 + off_ld + col];
-    if constexpr (VAR == TB_FIXED_CURVATURE) slq = psi[off_ld + col];
+    XF[IN] = (Hv)xin[off_ld + col];
+    XP[IN] = (Hv)xin[N3 + off_ld + col];
+    if constexpr (VAR == TB_FIXED_CURVATURE) PS[IN] = psi[off_ld + col];
 
-    buf[0][r][cc] = (T)xf1;
-    buf[1][r][cc] = (T)y1f1;
-    buf[2][r][cc] = (T)y2f1;
-    buf[3][r][cc] = (T)y3f1;
+    buf[0][r][cc] = (T)XF[I1];
+    buf[1][r][cc] = (T)Y1F[I1];
+    buf[2][r][cc] = (T)Y2F[I1];
+    buf[3][r][cc] = (T)Y3F[I1];
     __syncthreads();
 
     // ---- stage 1 at P-1: k1 = f(x)
-    double kf = (double)xp1;
-    double kp = tb_force<VAR, LAM>((double)xf1, (double)xf0, (double)xf2, (double)buf[0][rp][cc],
-                                  (double)buf[0][rm][cc], cnext(xf1, buf[0]), cprev(xf1, buf[0]),
-                                  (double)xp1, k.c_h[0], k.inv[0], k, ps1);
-    const Hv y1f0 = (Hv)rnd<F32C>((double)xf1 + k.dh * kf);
-    const Hv y1p0 = (Hv)rnd<F32C>((double)xp1 + k.dh * kp);
-    Hv af0, ap0;
+    double kf = (double)XP[I1];
+    double kp = tb_force<VAR, LAM>((double)XF[I1], (double)XF[I0], (double)XF[I2], (double)buf[0][rp][cc],
+                                  (double)buf[0][rm][cc], cnext(XF[I1], buf[0]), cprev(XF[I1], buf[0]),
+                                  (double)XP[I1], k.c_h[0], k.inv[0], k, PS[I1]);
+    Y1F[I0] = (Hv)rnd<F32C>((double)XF[I1] + k.dh * kf);
+    Y1P[I0] = (Hv)rnd<F32C>((double)XP[I1] + k.dh * kp);
     if constexpr (F32C) {
-      af0 = (Hv)(k.dt6 * kf);
-      ap0 = (Hv)(k.dt6 * kp);
+      AF[I0] = (Hv)(k.dt6 * kf);
+      AP[I0] = (Hv)(k.dt6 * kp);
     } else {
-      af0 = xf1 + k.dt6 * kf;
-      ap0 = xp1 + k.dt6 * kp;
+      AF[I0] = XF[I1] + k.dt6 * kf;
+      AP[I0] = XP[I1] + k.dt6 * kp;
     }
 
     // ---- stage 2 at P-2: k2 = f(y1)
-    kf = (double)y1p1;
-    kp = tb_force<VAR, LAM>((double)y1f1, (double)y1f0, (double)y1f2, (double)buf[1][rp][cc],
-                                  (double)buf[1][rm][cc], cnext(y1f1, buf[1]), cprev(y1f1, buf[1]),
-                                  (double)y1p1, k.c_h[1], k.inv[1], k, ps2);
-    const Hv y2f0 = (Hv)rnd<F32C>((double)xf2 + k.dh * kf);
-    const Hv y2p0 = (Hv)rnd<F32C>((double)xp2 + k.dh * kp);
-    af1 = (Hv)((double)af1 + k.dt3 * kf);
-    ap1 = (Hv)((double)ap1 + k.dt3 * kp);
+    kf = (double)Y1P[I1];
+    kp = tb_force<VAR, LAM>((double)Y1F[I1], (double)Y1F[I0], (double)Y1F[I2], (double)buf[1][rp][cc],
+                            (double)buf[1][rm][cc], cnext(Y1F[I1], buf[1]), cprev(Y1F[I1], buf[1]),
+                            (double)Y1P[I1], k.c_h[1], k.inv[1], k, PS[I2]);
+    Y2F[I0] = (Hv)rnd<F32C>((double)XF[I2] + k.dh * kf);
+    Y2P[I0] = (Hv)rnd<F32C>((double)XP[I2] + k.dh * kp);
+    AF[I1] = (Hv)((double)AF[I1] + k.dt3 * kf);
+    AP[I1] = (Hv)((double)AP[I1] + k.dt3 * kp);
 
     // ---- stage 3 at P-3: k3 = f(y2)
-    kf = (double)y2p1;
-    kp = tb_force<VAR, LAM>((double)y2f1, (double)y2f0, (double)y2f2, (double)buf[2][rp][cc],
-                                  (double)buf[2][rm][cc], cnext(y2f1, buf[2]), cprev(y2f1, buf[2]),
-                                  (double)y2p1, k.c_h[2], k.inv[2], k, ps3);
-    const Hv y3f0 = (Hv)rnd<F32C>((double)xf3 + k.dt * kf);
-    const Hv y3p0 = (Hv)rnd<F32C>((double)xp3 + k.dt * kp);
-    af2 = (Hv)((double)af2 + k.dt3 * kf);
-    ap2 = (Hv)((double)ap2 + k.dt3 * kp);
+    kf = (double)Y2P[I1];
+    kp = tb_force<VAR, LAM>((double)Y2F[I1], (double)Y2F[I0], (double)Y2F[I2], (double)buf[2][rp][cc],
+                            (double)buf[2][rm][cc], cnext(Y2F[I1], buf[2]), cprev(Y2F[I1], buf[2]),
+                            (double)Y2P[I1], k.c_h[2], k.inv[2], k, PS[I3]);
+    Y3F[I0] = (Hv)rnd<F32C>((double)XF[I3] + k.dt * kf);
+    Y3P[I0] = (Hv)rnd<F32C>((double)XP[I3] + k.dt * kp);
+    AF[I2] = (Hv)((double)AF[I2] + k.dt3 * kf);
+    AP[I2] = (Hv)((double)AP[I2] + k.dt3 * kp);
 
     // ---- stage 4 at P-4: k4 = f(y3); new state
-    kf = (double)y3p1;
-    kp = tb_force<VAR, LAM>((double)y3f1, (double)y3f0, (double)y3f2, (double)buf[3][rp][cc],
-                                  (double)buf[3][rm][cc], cnext(y3f1, buf[3]), cprev(y3f1, buf[3]),
-                                  (double)y3p1, k.c_h[3], k.inv[3], k, ps4);
+    kf = (double)Y3P[I1];
+    kp = tb_force<VAR, LAM>((double)Y3F[I1], (double)Y3F[I0], (double)Y3F[I2], (double)buf[3][rp][cc],
+                            (double)buf[3][rm][cc], cnext(Y3F[I1], buf[3]), cprev(Y3F[I1], buf[3]),
+                            (double)Y3P[I1], k.c_h[3], k.inv[3], k, PS[I4]);
     if constexpr (F32C) {
-      const double s0 = ((double)xf4 + lo_f) + ((double)af3 + k.dt6 * kf);
-      const double s1 = ((double)xp4 + lo_p) + ((double)ap3 + k.dt6 * kp);
+      const double s0 = ((double)XF[I4] + lo_f) + ((double)AF[I3] + k.dt6 * kf);
+      const double s1 = ((double)XP[I4] + lo_p) + ((double)AP[I3] + k.dt6 * kp);
       const float h0 = (float)s0, h1 = (float)s1;
       q.f = h0;
       q.p = h1;
       q.lf = (float)(s0 - (double)h0);
       q.lp = (float)(s1 - (double)h1);
     } else {
-      q.f = af3 + k.dt6 * kf;
-      q.p = ap3 + k.dt6 * kp;
+      q.f = AF[I3] + k.dt6 * kf;
+      q.p = AP[I3] + k.dt6 * kp;
     }
     q.off = st_off;
     q.plane = off_st;
     if constexpr (!DEFER) flush(q);
     off_st = next_plane(off_st);
-
-    // ---- advance the histories by one plane
-    xf4 = xf3; xf3 = xf2; xf2 = xf1; xf1 = xf0;
-    ps4 = ps3; ps3 = ps2; ps2 = ps1; ps1 = ps0;
-    xp4 = xp3; xp3 = xp2; xp2 = xp1; xp1 = xp0;
-    y1f2 = y1f1; y1f1 = y1f0; y1p1 = y1p0;
-    y2f2 = y2f1; y2f1 = y2f0; y2p1 = y2p0;
-    y3f2 = y3f1; y3f1 = y3f0; y3p1 = y3p0;
-    af3 = af2; af2 = af1; af1 = af0;
-    ap3 = ap2; ap2 = ap1; ap1 = ap0;
   };
-  int j = 0;
-  for (; j + 1 < iters; j += 2) {
-    iteration(j, s0f, s0p, s0q, l0f, l0p, lds[0], q0);
-    iteration(j + 1, s1f, s1p, s1q, l1f, l1p, lds[1], q1);
+  using std::integral_constant;
+  if constexpr (RING) {
+    for (int j = 0; j < iters; j += TBU) {
+      iteration(integral_constant<int, 0>{}, integral_constant<int, 0>{}, j);
+      iteration(integral_constant<int, 1>{}, integral_constant<int, 1>{}, j + 1);
+      iteration(integral_constant<int, 2>{}, integral_constant<int, 0>{}, j + 2);
+      iteration(integral_constant<int, 3>{}, integral_constant<int, 1>{}, j + 3);
+      iteration(integral_constant<int, 4>{}, integral_constant<int, 0>{}, j + 4);
+      iteration(integral_constant<int, 5>{}, integral_constant<int, 1>{}, j + 5);
+      iteration(integral_constant<int, 6>{}, integral_constant<int, 0>{}, j + 6);
+      iteration(integral_constant<int, 7>{}, integral_constant<int, 1>{}, j + 7);
+    }
+  } else {
+    // new[i] = old[i + 1]: the entry for plane P+1 becomes the entry for P
+    auto rotate = [](auto &a) {
+      const auto t = a[0];
+#pragma unroll
+      for (int i = 0; i + 1 < TBU; ++i) a[i] = a[i + 1];
+      a[TBU - 1] = t;
+    };
+    auto advance = [&]() {
+      rotate(XF);
+      rotate(XP);
+      rotate(Y1F);
+      rotate(Y1P);
+      rotate(Y2F);
+      rotate(Y2P);
+      rotate(Y3F);
+      rotate(Y3P);
+      rotate(AF);
+      rotate(AP);
+      if constexpr (VAR == TB_FIXED_CURVATURE) rotate(PS);
+    };
+    int j = 0;
+    for (; j + 1 < iters; j += 2) {
+      iteration(integral_constant<int, 0>{}, integral_constant<int, 0>{}, j);
+      advance();
+      iteration(integral_constant<int, 0>{}, integral_constant<int, 1>{}, j + 1);
+      advance();
+    }
+    if (j < iters) iteration(integral_constant<int, 0>{}, integral_constant<int, 0>{}, j);
   }
-  if (j < iters) iteration(j, s0f, s0p, s0q, l0f, l0p, lds[0], q0);
   if constexpr (DEFER) {
-    flush(q0);
-    flush(q1);
+    flush(Q[0]);
+    flush(Q[1]);
   }
 }
 
