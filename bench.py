@@ -171,7 +171,7 @@ def ensemble_workload(args):
         "ms_per_step": ms / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "storage": "f32 hi + f32 compensation", "data": "synthetic (Gaussian random fields, seeds 100-195)",
         "config": {"workload": "ensemble 32xN=64 + 32xN=96 + 32xN=128, L=N, FRW KG m=1 lambda=0.1, dt=0.05",
-                   "sites": ens.sites, "tuned_grid": tuned_grid, "tuned_chunks": plan, "graph_wall_s": wall},
+                   "sites": ens.sites, "tuned_grid": "flat (one block per unit)" if tuned_grid == cfl.CF_ENS_GRID_FLAT else tuned_grid, "tuned_chunks": plan, "graph_wall_s": wall},
         "ensemble_avg_density_spectra": summary,
     }
     print(json.dumps(line), flush=True)

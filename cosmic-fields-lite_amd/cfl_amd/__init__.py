@@ -12,7 +12,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import (CF_PREC_F32C, CF_PREC_F64, CF_SPEC_DERIV_POWER_CUTOFF, CF_SPEC_GAUSS,  # noqa: F401
+from ._lib import (CF_ENS_GRID_FLAT, CF_PREC_F32C, CF_PREC_F64, CF_SPEC_DERIV_POWER_CUTOFF, CF_SPEC_GAUSS,  # noqa: F401
                    CF_SPEC_KM3_GAUSS, CF_SPEC_POWER_CUTOFF, CF_SPEC_SCALE_INV,
                    CF_VAR_APPROX_COMOVING_CURVATURE, CF_VAR_COMOVING_CURVATURE, CF_VAR_FIXED_CURVATURE,
                    CF_VAR_SQRT_POTENTIAL, CfError, Cosmology, KGParams, check)

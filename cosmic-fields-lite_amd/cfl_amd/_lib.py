@@ -21,6 +21,8 @@ CF_E_UNSUPPORTED = 5
 CF_PREC_F64 = 0
 CF_PREC_F32C = 1
 
+CF_ENS_GRID_FLAT = -1  # ensemble plan: one block per work unit
+
 CF_VAR_FIXED_CURVATURE = 1
 CF_VAR_SQRT_POTENTIAL = 2
 CF_VAR_COMOVING_CURVATURE = 3

@@ -59,6 +59,53 @@ __device__ __forceinline__ V wave_uniform(const V &v) {
   return __builtin_bit_cast(V, a);
 }
 
+// One work unit of the step: its lattice's step scalars, then the temporally
+// blocked body.
+template <class T, bool F32C, bool LAM, bool DEFER, bool LANE_NBR = true>
+__device__ __forceinline__ void ens_unit(TbLds<T> &lds, const EnsReal *__restrict__ reals, const EnsUnit &unit,
+                                         const EnsStepCoef &sc, int parity, double c_m, double lam, double dt) {
+  const EnsUnit e = wave_uniform(unit);
+  const EnsReal &R = reals[e.real];
+  // Select this step's buffers first, then make each pointer uniform: indexing
+  // x[parity] after the struct is in registers leaves the compiler treating the
+  // pointers as divergent (plane addresses then run on the vector unit and every
+  // buffer-resource build re-reads them lane by lane).
+  const T *xin = wave_uniform((const T *)(parity ? R.x[1] : R.x[0]));
+  T *xout = wave_uniform((T *)(parity ? R.x[0] : R.x[1]));
+  const float *lin = wave_uniform(parity ? R.lo[1] : R.lo[0]);
+  float *lout = wave_uniform(parity ? R.lo[0] : R.lo[1]);
+  const int64_t N = wave_uniform(R.N);
+  const double L = wave_uniform(R.L);
+  TbStep k;
+  // The step scalars are the same for the whole block; moving them to scalar
+  // registers keeps the vector registers for the step body.
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    k.c_h[s] = wave_uniform((-3.0) * sc.H[s]);
+    k.inv[s] = wave_uniform(1.0 / ((L / N) * (L / N)) / (sc.a[s] * sc.a[s]));  // cf_inv_ah_sqr
+  }
+  k.c_m = c_m;
+  k.lam = lam;
+  k.m = 0.0;
+  k.fa2 = 0.0;
+  k.dh = dt / 2.0;
+  k.dt = dt;
+  k.dt6 = dt / 6.0;
+  k.dt3 = dt / 3.0;
+  k = wave_uniform(k);  // the divisions above ran on the vector unit (no scalar FP64)
+  TbUnit u;
+  u.N = N;
+  u.b0 = e.b0;
+  u.c0 = e.c0;
+  u.a0 = e.a0;
+  u.a1 = e.a1;
+  u.T = e.T;
+  tb_unit<T, F32C, LAM, DEFER, LANE_NBR>(lds, u, k, xin, lin, xout, lout);
+}
+
+// Persistent form: a fixed grid; block b runs units unit_off[b] .. unit_off[b+1]-1
+// (an LPT-balanced list). Stores are not deferred: the unit loop leaves less
+// register room than a single unit.
 template <class T, bool F32C, bool LAM>
 __global__ __launch_bounds__(1024) void k_ens_tb(const EnsReal *__restrict__ reals,
                                                  const int32_t *__restrict__ unit_off,
@@ -69,32 +116,20 @@ __global__ __launch_bounds__(1024) void k_ens_tb(const EnsReal *__restrict__ rea
   const EnsStepCoef sc = steps[step];
   const int beg = unit_off[blockIdx.x], end = unit_off[blockIdx.x + 1];
   for (int i = beg; i < end; ++i) {
-    const EnsUnit e = wave_uniform(units[i]);
-    const EnsReal R = wave_uniform(reals[e.real]);
-    TbStep k;
-    // The step scalars are the same for the whole block; moving them to scalar
-    // registers keeps the vector registers for the step body.
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      k.c_h[s] = wave_uniform((-3.0) * sc.H[s]);
-      k.inv[s] = wave_uniform(1.0 / ((R.L / R.N) * (R.L / R.N)) / (sc.a[s] * sc.a[s]));  // cf_inv_ah_sqr
-    }
-    k.c_m = c_m;
-    k.lam = lam;
-    k.dh = dt / 2.0;
-    k.dt = dt;
-    k.dt6 = dt / 6.0;
-    k.dt3 = dt / 3.0;
-    TbUnit u;
-    u.N = R.N;
-    u.b0 = e.b0;
-    u.c0 = e.c0;
-    u.a0 = e.a0;
-    u.a1 = e.a1;
-    u.T = e.T;
-    tb_unit<T, F32C, LAM, false>(lds, u, k, (const T *)R.x[parity], R.lo[parity], (T *)R.x[1 - parity], R.lo[1 - parity]);
+    ens_unit<T, F32C, LAM, false>(lds, reals, units[i], sc, parity, c_m, lam, dt);
     __syncthreads();  // the next unit rewrites the plane buffers
   }
+}
+
+// Flat form: one block per unit (units sorted longest first, so the hardware
+// dispatcher balances the tail), the body of the single-lattice kernel.
+template <class T, bool F32C, bool LAM>
+__global__ __launch_bounds__(1024) void k_ens_tb_flat(const EnsReal *__restrict__ reals,
+                                                      const EnsUnit *__restrict__ units,
+                                                      const EnsStepCoef *__restrict__ steps, int64_t step,
+                                                      int parity, double c_m, double lam, double dt) {
+  __shared__ TbLds<T> lds;
+  ens_unit<T, F32C, LAM, true, !F32C>(lds, reals, units[blockIdx.x], steps[step], parity, c_m, lam, dt);
 }
 
 }  // namespace cf
@@ -165,10 +200,20 @@ static cf_status ens_build(cf_ensemble *e) {
           cost.push_back((int)(u.a1 - u.a0) + 8);
         }
   }
-  const int G = std::max(1, std::min<int>(e->grid, (int)units.size()));
   std::vector<int> order(units.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  if (e->grid == CF_ENS_GRID_FLAT) {  // one block per unit, longest first
+    std::vector<EnsUnit> flat;
+    for (int i : order) flat.push_back(units[i]);
+    ens_free_plan(e);
+    e->n_units = (int)flat.size();
+    CF_HIP(hipMalloc((void **)&e->d_units, flat.size() * sizeof(EnsUnit)));
+    CF_HIP(hipMemcpy(e->d_units, flat.data(), flat.size() * sizeof(EnsUnit), hipMemcpyHostToDevice));
+    ens_drop_graph(e);
+    return CF_OK;
+  }
+  const int G = std::max(1, std::min<int>(e->grid, (int)units.size()));
   typedef std::pair<long, int> Load;  // (load, block)
   std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
   for (int b = 0; b < G; ++b) heap.push(Load(0, b));
@@ -222,6 +267,26 @@ static cf_status ens_launch(cf_ensemble *e, int64_t step, int parity, double dt,
   dim3 grd(e->grid), blk(1024);
   const double c_m = e->m * e->m;
   const bool lam = e->lambda != 0.0;
+  if (e->grid == CF_ENS_GRID_FLAT) {
+    grd = dim3(e->n_units);
+    if (e->prec == CF_PREC_F32C) {
+      if (lam)
+        hipLaunchKernelGGL((k_ens_tb_flat<float, true, true>), grd, blk, 0, st, e->d_reals, e->d_units, e->d_steps,
+                           step, parity, c_m, e->lambda, dt);
+      else
+        hipLaunchKernelGGL((k_ens_tb_flat<float, true, false>), grd, blk, 0, st, e->d_reals, e->d_units, e->d_steps,
+                           step, parity, c_m, e->lambda, dt);
+    } else {
+      if (lam)
+        hipLaunchKernelGGL((k_ens_tb_flat<double, false, true>), grd, blk, 0, st, e->d_reals, e->d_units,
+                           e->d_steps, step, parity, c_m, e->lambda, dt);
+      else
+        hipLaunchKernelGGL((k_ens_tb_flat<double, false, false>), grd, blk, 0, st, e->d_reals, e->d_units,
+                           e->d_steps, step, parity, c_m, e->lambda, dt);
+    }
+    CF_LAUNCHED();
+    return CF_OK;
+  }
   if (e->prec == CF_PREC_F32C) {
     if (lam)
       hipLaunchKernelGGL((k_ens_tb<float, true, true>), grd, blk, 0, st, e->d_reals, e->d_unit_off, e->d_units,
@@ -372,7 +437,8 @@ cf_status cf_ensemble_set_plan(cf_ensemble *e, int64_t N, int chunk, int grid) {
     CF_REQUIRE(e->chunk.count(N), "cf_ensemble_set_plan: no lattice of size %lld", (long long)N);
     e->chunk[N] = (chunk <= 0 || chunk > N) ? (int)N : chunk;
   }
-  if (grid > 0) e->grid = grid;
+  CF_REQUIRE(grid >= 0 || grid == CF_ENS_GRID_FLAT, "cf_ensemble_set_plan: bad grid %d", grid);
+  if (grid != 0) e->grid = grid;
   return ens_build(e);
 }
 
@@ -405,7 +471,7 @@ cf_status cf_ensemble_tune(cf_ensemble *e, double dt, int reps, int *grid_out) {
   CF_REQUIRE(e, "cf_ensemble_tune: null ensemble");
   if (reps <= 0) reps = 3;
   CF_TRY(ens_fill_steps(e, 0.0, 0, 1, dt));
-  const int grids[] = {256, 512, 768, 1024, 2048};
+  const int grids[] = {256, 512, 768, 1024, 2048, CF_ENS_GRID_FLAT};
   float best = 1e30f, ms = 0;
   int best_g = e->grid;
   for (int g : grids) {

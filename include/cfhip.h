@@ -163,10 +163,14 @@ cf_status cf_ensemble_state(cf_ensemble *e, int r, void **x_dev, float **lo_dev)
 /* Initial conditions: realisation r gets the GRF of seeds[r] (cf_init_grf_state). */
 cf_status cf_ensemble_init_grf(cf_ensemble *e, const uint32_t *seeds, int kind, const double *params, double rms,
                                cf_stream stream);
-/* Work plan: planes per unit for lattices of size N (N <= 0: keep) and the grid. */
+/* Work plan: planes per unit for lattices of size N (N <= 0: keep) and the grid:
+ * > 0: that many persistent blocks, each running an LPT-balanced list of units;
+ * CF_ENS_GRID_FLAT: one block per unit, longest first (the dispatcher balances);
+ * 0: keep. */
+#define CF_ENS_GRID_FLAT (-1)
 cf_status cf_ensemble_set_plan(cf_ensemble *e, int64_t N, int chunk, int grid);
 cf_status cf_ensemble_get_plan(const cf_ensemble *e, int64_t N, int *chunk, int *grid);
-/* Autotuner: times candidate grids, then candidate chunks per N, on the device
+/* Autotuner: times candidate grids (persistent and flat), then candidate chunks per N, on the device
  * (a step reads one buffer and writes the other, so the state is untouched), and
  * keeps the fastest plan. Blocking. */
 cf_status cf_ensemble_tune(cf_ensemble *e, double dt, int reps, int *grid_out);

@@ -41,10 +41,14 @@ def make(cfl, prec, lam=0.1):
     return ens
 
 
-@pytest.mark.parametrize("prec", [0, 1])
-def test_ensemble_equals_single_lattice_runs(cfl, prec):
-    print(f"case ensemble prec={prec}", flush=True)
+@pytest.mark.parametrize("prec,grid", [(0, 0), (1, 0), (0, -1), (1, -1)])
+def test_ensemble_equals_single_lattice_runs(cfl, prec, grid):
+    """Persistent (default grid) and flat (grid = CF_ENS_GRID_FLAT) launches."""
+    print(f"case ensemble prec={prec} grid={grid}", flush=True)
     ens = make(cfl, prec)
+    if grid:
+        ens.set_plan(0, 0, grid)
+        assert ens.get_plan(NS[0])[1] == grid
     t0, k0, steps, dt = 1.0, 2, 3, 0.05
     init = [raw_state(cfl, ens, r) for r in range(ens.n)]
     ens.run(t0, k0, steps, dt)
@@ -76,11 +80,13 @@ def test_ensemble_oracle_small(cfl, O):
     ens.close()
 
 
-@pytest.mark.parametrize("prec", [0, 1])
-def test_graph_replays_equal_eager(cfl, prec):
-    print(f"case graph prec={prec}", flush=True)
+@pytest.mark.parametrize("prec,grid", [(0, 0), (1, 0), (1, -1)])
+def test_graph_replays_equal_eager(cfl, prec, grid):
+    print(f"case graph prec={prec} grid={grid}", flush=True)
     seeds = list(range(100, 100 + len(NS)))
     ens = make(cfl, prec)
+    if grid:
+        ens.set_plan(0, 0, grid)
     ens.run(1.0, 0, 4, 0.05, use_graph=False)
     torch.cuda.synchronize()
     eager = [host(ens.state(r)) for r in range(ens.n)]
@@ -99,7 +105,7 @@ def test_autotuner_keeps_bits(cfl):
     ens = make(cfl, 1)
     x0 = [raw_state(cfl, ens, r) for r in range(ens.n)]
     g = ens.tune(0.05, reps=2)
-    assert g > 0
+    assert g > 0 or g == cfl.CF_ENS_GRID_FLAT
     # tuning did not touch the state
     for r in range(ens.n):
         for a, b in zip(raw_state(cfl, ens, r), x0[r]):
