@@ -34,7 +34,7 @@
 #include <cmath>
 #include <vector>
 
-#include "cf_common.hpp"
+#include "cf_march.hpp"
 
 namespace cf {
 
@@ -43,15 +43,16 @@ constexpr int ADJ_THREADS = 256;
 
 __device__ __forceinline__ int64_t wrapi(int64_t i, int64_t N) { return i < 0 ? i + N : (i >= N ? i - N : i); }
 
-__device__ __forceinline__ void block_partial(double v, double *part_slot) {
+// Fixed-order tree sum of one value per thread (tid = linear thread index).
+__device__ __forceinline__ void block_partial(double v, double *part_slot, int tid) {
   __shared__ double sm[ADJ_THREADS];
-  sm[threadIdx.x] = v;
+  sm[tid] = v;
   __syncthreads();
   for (int s = ADJ_THREADS / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    if (tid < s) sm[tid] += sm[tid + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *part_slot = sm[0];
+  if (tid == 0) *part_slot = sm[0];
 }
 
 struct AdjCoef {
@@ -64,44 +65,99 @@ struct AdjCoef {
   double dk;
 };
 
-// One backward stage: a = ck*lp + dk*yb_prev; out = J(y)^T a;
-// mode 0: acc = lp + out, ybar = out; mode 1: acc += out, ybar = out; mode 2: acc += out.
-// part[blockIdx] = sum over the block's sites of (-2 m phi_y) a_pi.
-__global__ __launch_bounds__(ADJ_THREADS) void k_adj_stage(int64_t N, const double *__restrict__ lp,
-                                                           const double *__restrict__ ybp,
-                                                           const double *__restrict__ y, double *__restrict__ ybar,
-                                                           double *__restrict__ acc, int mode, AdjCoef k,
-                                                           double *__restrict__ part) {
-  const int64_t NN = N * N, N3 = NN * N;
-  double dm = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * ADJ_THREADS + threadIdx.x; i < N3; i += (int64_t)ADJ_BLOCKS * ADJ_THREADS) {
-    const int64_t a = i / NN, b = (i / N) % N, c = i % N;
-    auto api = [&](int64_t j) {
-      const double v = k.ck * lp[N3 + j];
-      return ybp ? v + k.dk * ybp[N3 + j] : v;
-    };
-    const double aphi = ybp ? k.ck * lp[i] + k.dk * ybp[i] : k.ck * lp[i];
-    const double a0 = api(i);
-    const double nb = api(wrapi(a + 1, N) * NN + b * N + c) + api(wrapi(a - 1, N) * NN + b * N + c) +
-                      api(a * NN + wrapi(b + 1, N) * N + c) + api(a * NN + wrapi(b - 1, N) * N + c) +
-                      api(a * NN + b * N + wrapi(c + 1, N)) + api(a * NN + b * N + wrapi(c - 1, N));
-    const double phi = y[i];
-    const double out_phi = (-k.c_m - 3.0 * k.lam * phi * phi) * a0 + k.inv * (nb - 6.0 * a0);
-    const double out_pi = aphi + k.c_h * a0;
-    if (mode == 0) {
-      acc[i] = lp[i] + out_phi;
-      acc[N3 + i] = lp[N3 + i] + out_pi;
-    } else {
-      acc[i] += out_phi;
-      acc[N3 + i] += out_pi;
-    }
-    if (mode != 2) {
-      ybar[i] = out_phi;
-      ybar[N3 + i] = out_pi;
-    }
-    dm += (-k.two_m * phi) * a0;
+// Stage-state recompute: y = x + cy * f(in) (the forward stage kernels' arithmetic,
+// so y1..y3 equal the forward step's stage states bit for bit). One thread per (b,c)
+// column marching along a, as the forward stage kernels.
+template <bool LAM>
+__global__ __launch_bounds__(ADJ_THREADS) void k_adj_y(Tiling tl, RhsCoef rk, const double *__restrict__ in,
+                                                       const double *__restrict__ x, double *__restrict__ y,
+                                                       double cy) {
+  int ct, bt, ac;
+  tile_of(tl, ct, bt, ac);
+  const int64_t N = tl.N, NN = N * N, N3 = NN * N;
+  const int64_t c = (int64_t)ct * TC + threadIdx.x;
+  const int64_t b = (int64_t)bt * TB + threadIdx.y;
+  const int64_t a0 = (int64_t)ac * tl.A;
+  if (c >= N || b >= N || a0 >= N) return;
+  const int64_t a1 = (a0 + tl.A < N) ? a0 + tl.A : N;
+  const int64_t cp = (c + 1 == N) ? 0 : c + 1, cm = (c == 0) ? N - 1 : c - 1;
+  const int64_t bp = (b + 1 == N) ? 0 : b + 1, bm = (b == 0) ? N - 1 : b - 1;
+  const int64_t col = b * N + c;
+  double prev = in[((a0 == 0) ? N - 1 : a0 - 1) * NN + col];
+  double cur = in[a0 * NN + col];
+  for (int64_t a = a0; a < a1; ++a) {
+    const int64_t ap = (a + 1 == N) ? 0 : a + 1;
+    const int64_t row = a * NN, i = row + col;
+    const double next = in[ap * NN + col];
+    const double piv = in[N3 + i];
+    const double kpi = rhs_pi(cur, next, prev, in[row + bp * N + c], in[row + bm * N + c], in[row + b * N + cp],
+                              in[row + b * N + cm], piv, rk, LAM);
+    y[i] = x[i] + cy * piv;
+    y[N3 + i] = x[N3 + i] + cy * kpi;
+    prev = cur;
+    cur = next;
   }
-  block_partial(dm, part + blockIdx.x);
+}
+
+// One backward stage: a = ck*lp + dk*yb_prev (HAS_YB) or ck*lp; out = J(y)^T a;
+// MODE 0: acc = lp + out, ybar = out; MODE 1: acc += out, ybar = out; MODE 2: acc += out.
+// part[blockIdx] = sum over the block's sites of (-2 m phi_y) a_pi, in a fixed order.
+// A fixed grid of ADJ_BLOCKS blocks walks the tiles (one thread per (b,c) column
+// marching along a, a_pi at a-1, a, a+1 kept in registers).
+template <int MODE, bool HAS_YB>
+__global__ __launch_bounds__(ADJ_THREADS) void k_adj_back(Tiling tl, const double *__restrict__ lp,
+                                                          const double *__restrict__ ybp,
+                                                          const double *__restrict__ y, double *__restrict__ ybar,
+                                                          double *__restrict__ acc, AdjCoef k,
+                                                          double *__restrict__ part) {
+  const int64_t N = tl.N, NN = N * N, N3 = NN * N;
+  auto api = [&](int64_t j) {
+    const double v = k.ck * lp[N3 + j];
+    if constexpr (HAS_YB) return v + k.dk * ybp[N3 + j];
+    else return v;
+  };
+  double dm = 0.0;
+  for (int t = blockIdx.x; t < tl.total; t += ADJ_BLOCKS) {
+    const int ct = t % tl.n_ct, r = t / tl.n_ct, bt = r % tl.n_bt, ac = r / tl.n_bt;
+    const int64_t c = (int64_t)ct * TC + threadIdx.x;
+    const int64_t b = (int64_t)bt * TB + threadIdx.y;
+    const int64_t a0 = (int64_t)ac * tl.A;
+    if (c >= N || b >= N || a0 >= N) continue;
+    const int64_t a1 = (a0 + tl.A < N) ? a0 + tl.A : N;
+    const int64_t cp = (c + 1 == N) ? 0 : c + 1, cm = (c == 0) ? N - 1 : c - 1;
+    const int64_t bp = (b + 1 == N) ? 0 : b + 1, bm = (b == 0) ? N - 1 : b - 1;
+    const int64_t col = b * N + c;
+    double prev = api(((a0 == 0) ? N - 1 : a0 - 1) * NN + col);
+    double cur = api(a0 * NN + col);
+    for (int64_t a = a0; a < a1; ++a) {
+      const int64_t ap = (a + 1 == N) ? 0 : a + 1;
+      const int64_t row = a * NN, i = row + col;
+      const double next = api(ap * NN + col);
+      double aphi = k.ck * lp[i];
+      if constexpr (HAS_YB) aphi = aphi + k.dk * ybp[i];
+      const double a0v = cur;
+      const double nb = next + prev + api(row + bp * N + c) + api(row + bm * N + c) + api(row + b * N + cp) +
+                        api(row + b * N + cm);
+      const double phi = y[i];
+      const double out_phi = (-k.c_m - 3.0 * k.lam * phi * phi) * a0v + k.inv * (nb - 6.0 * a0v);
+      const double out_pi = aphi + k.c_h * a0v;
+      if constexpr (MODE == 0) {
+        acc[i] = lp[i] + out_phi;
+        acc[N3 + i] = lp[N3 + i] + out_pi;
+      } else {
+        acc[i] += out_phi;
+        acc[N3 + i] += out_pi;
+      }
+      if constexpr (MODE != 2) {
+        ybar[i] = out_phi;
+        ybar[N3 + i] = out_pi;
+      }
+      dm += (-k.two_m * phi) * a0v;
+      prev = cur;
+      cur = next;
+    }
+  }
+  block_partial(dm, part + blockIdx.x, (int)(threadIdx.y * TC + threadIdx.x));
 }
 
 // Half-spectrum fk *= 2 w[s].
@@ -141,7 +197,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_rho_adjoint(int64_t N, const do
     gx[N3 + i] = g[i] * x[N3 + i];
     dm += g[i] * m * phi * phi;
   }
-  block_partial(dm, part + blockIdx.x);
+  block_partial(dm, part + blockIdx.x, (int)threadIdx.x);
 }
 
 static double host_sum(const std::vector<double> &v, size_t beg, size_t n) {
@@ -229,20 +285,30 @@ cf_status cf_rk4_step_adjoint_f64(const cf_kg_params *p, const double *x, double
   CF_REQUIRE(valid_N(p->N), "cf_rk4_step_adjoint_f64: unsupported N");
   const int64_t N = p->N, n = 2 * N * N * N;
   hipStream_t st = S(stream);
-  double *y1 = (double *)work_dev, *y2 = y1 + n, *y3 = y2 + n, *kk = y3 + n;
-  double *yb0 = kk + n, *yb1 = yb0 + n, *acc = yb1 + n;
+  double *y1 = (double *)work_dev, *y2 = y1 + n, *y3 = y2 + n;
+  double *yb0 = y3 + 2 * n, *yb1 = yb0 + n, *acc = yb1 + n;  // (y3 + n: unused)
   const double dh = dt / 2.0, th = t + dh, dt6 = dt / 6.0, dt3 = dt / 3.0;
-  // recompute the stage states with the forward kernels (same bits as the step)
-  auto rhs = [&](const double *xx, double tt, double *out) {
+  const Tiling tl = make_tiling(N, default_chunk(N));
+  const dim3 blk(TC, TB);
+  static_assert(TC * TB == ADJ_THREADS, "block shape");
+  // recompute the stage states with the forward stage arithmetic (same bits)
+  auto ystage = [&](const double *in, double tt, double cy, double *y) -> cf_status {
     const double a_t = cf_cosmo_a(&p->cosmo, tt);
-    return cf_rhs_f64(N, xx, out, p->m, p->lambda, cf_cosmo_H(&p->cosmo, tt), cf_inv_ah_sqr(N, p->L, a_t), stream);
+    RhsCoef rk;
+    rk.c_h = (-3.0) * cf_cosmo_H(&p->cosmo, tt);
+    rk.c_m = p->m * p->m;
+    rk.lam = p->lambda;
+    rk.inv = cf_inv_ah_sqr(N, p->L, a_t);
+    if (rk.lam != 0.0)
+      hipLaunchKernelGGL(k_adj_y<true>, dim3(tl.total), blk, 0, st, tl, rk, in, x, y, cy);
+    else
+      hipLaunchKernelGGL(k_adj_y<false>, dim3(tl.total), blk, 0, st, tl, rk, in, x, y, cy);
+    CF_LAUNCHED();
+    return CF_OK;
   };
-  CF_TRY(rhs(x, t, kk));
-  CF_TRY(cf_scale_sum2(y1, x, kk, 1.0, dh, n, stream));
-  CF_TRY(rhs(y1, th, kk));
-  CF_TRY(cf_scale_sum2(y2, x, kk, 1.0, dh, n, stream));
-  CF_TRY(rhs(y2, th, kk));
-  CF_TRY(cf_scale_sum2(y3, x, kk, 1.0, dt, n, stream));
+  CF_TRY(ystage(x, t, dh, y1));
+  CF_TRY(ystage(y1, th, dh, y2));
+  CF_TRY(ystage(y2, th, dt, y3));
   auto coef = [&](double tt, double ck, double dk) {
     AdjCoef k;
     const double a_t = cf_cosmo_a(&p->cosmo, tt);
@@ -255,14 +321,15 @@ cf_status cf_rk4_step_adjoint_f64(const cf_kg_params *p, const double *x, double
     k.dk = dk;
     return k;
   };
-  dim3 g(ADJ_BLOCKS), b(ADJ_THREADS);
-  hipLaunchKernelGGL(k_adj_stage, g, b, 0, st, N, lam, (const double *)nullptr, y3, yb0, acc, 0, coef(t + dt, dt6, 0.0),
-                     part_dev);
-  hipLaunchKernelGGL(k_adj_stage, g, b, 0, st, N, lam, yb0, y2, yb1, acc, 1, coef(th, dt3, dt), part_dev + ADJ_BLOCKS);
-  hipLaunchKernelGGL(k_adj_stage, g, b, 0, st, N, lam, yb1, y1, yb0, acc, 1, coef(th, dt3, dh),
+  const dim3 g(ADJ_BLOCKS);
+  hipLaunchKernelGGL((k_adj_back<0, false>), g, blk, 0, st, tl, lam, (const double *)nullptr, y3, yb0, acc,
+                     coef(t + dt, dt6, 0.0), part_dev);
+  hipLaunchKernelGGL((k_adj_back<1, true>), g, blk, 0, st, tl, lam, yb0, y2, yb1, acc, coef(th, dt3, dt),
+                     part_dev + ADJ_BLOCKS);
+  hipLaunchKernelGGL((k_adj_back<1, true>), g, blk, 0, st, tl, lam, yb1, y1, yb0, acc, coef(th, dt3, dh),
                      part_dev + 2 * ADJ_BLOCKS);
-  hipLaunchKernelGGL(k_adj_stage, g, b, 0, st, N, lam, yb0, x, (double *)nullptr, acc, 2, coef(t, dt6, dh),
-                     part_dev + 3 * ADJ_BLOCKS);
+  hipLaunchKernelGGL((k_adj_back<2, true>), g, blk, 0, st, tl, lam, yb0, x, (double *)nullptr, acc,
+                     coef(t, dt6, dh), part_dev + 3 * ADJ_BLOCKS);
   CF_LAUNCHED();
   CF_HIP(hipMemcpyAsync(lam, acc, n * sizeof(double), hipMemcpyDeviceToDevice, st));
   return CF_OK;

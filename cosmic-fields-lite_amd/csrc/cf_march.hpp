@@ -47,6 +47,24 @@ static inline Tiling make_tiling(int64_t N, int A) {
 
 static inline int default_chunk(int64_t N) { return N >= 256 ? 32 : (N >= 64 ? 16 : (int)N); }
 
+// Right-hand side of the pi equation at one site (KG_FRW_equation_kernel /
+// Lambda_FRW_equation_kernel, equations_cuda.cu:15-61, in that operation order).
+struct RhsCoef {
+  double c_h;  // (-3.0) * H(t)
+  double c_m;  // m * m
+  double lam;  // lambda
+  double inv;  // inv_ah_sqr(t)
+};
+
+__device__ __forceinline__ double rhs_pi(double phi, double pa1, double pam, double pb1, double pbm,
+                                         double pc1, double pcm, double pi, const RhsCoef &k,
+                                         bool has_lam) {
+  double v = k.c_h * pi - k.c_m * phi;
+  if (has_lam) v = v - ((k.lam * phi) * phi) * phi;
+  double r = v + k.inv * ((((-6.0 * phi + pa1) + pam) + pb1) + pbm);
+  return r + k.inv * (pc1 + pcm);
+}
+
 // Stage-specific inputs and outputs. T is the storage type (double, or float
 // for the compensated mode); all arithmetic is double.
 template <class T>

@@ -24,22 +24,6 @@
 
 namespace cf {
 
-struct RhsCoef {
-  double c_h;  // (-3.0) * H(t)
-  double c_m;  // m * m
-  double lam;  // lambda
-  double inv;  // inv_ah_sqr(t)
-};
-
-__device__ __forceinline__ double rhs_pi(double phi, double pa1, double pam, double pb1, double pbm,
-                                         double pc1, double pcm, double pi, const RhsCoef &k,
-                                         bool has_lam) {
-  double v = k.c_h * pi - k.c_m * phi;
-  if (has_lam) v = v - ((k.lam * phi) * phi) * phi;
-  double r = v + k.inv * ((((-6.0 * phi + pa1) + pam) + pb1) + pbm);
-  return r + k.inv * (pc1 + pcm);
-}
-
 // MODE 0: unfused RHS (dxdt -> yout); MODE 1: FP64 fused stage; MODE 2: F32C fused stage.
 template <class T, int MODE, int S, bool LAM>
 __global__ __launch_bounds__(256) void k_march(Tiling tl, RhsCoef rk, StageIO<T> io) {
