@@ -26,8 +26,9 @@ struct cf_spectrum_plan {
   double *rbuf = nullptr;        // N^3 real scratch
   double *cbuf0 = nullptr;       // half-spectrum scratch (2*nhalf doubles)
   double *cbuf1 = nullptr;       // second half-spectrum scratch
-  int64_t *bin_off = nullptr;    // nb+1 offsets into bin_idx
-  uint32_t *bin_idx = nullptr;   // nhalf entry indices sorted by bin
+  int64_t nr = 0;                // row classes r = as^2 + bs^2 <= 2 (N/2)^2
+  int32_t *roff = nullptr;       // nr+1 offsets into rows
+  int32_t *rows = nullptr;       // the N^2 rows (a*N + b) of the half spectrum sorted by r
   double *red = nullptr;         // reduction scratch (RED_BLOCKS + 1 doubles)
 };
 
@@ -45,54 +46,81 @@ constexpr int RED_BLOCKS = 1024;
     }                                                                       \
   } while (0)
 
-__device__ __forceinline__ double wave_sum(double v) {
-  // fixed butterfly over the 64 lanes of a wavefront
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+// Binning (fdm3d.cpp:11-28 sums |f_k|^2 over the full k-cube by s = |n|^2; the half
+// spectrum counts interior c columns twice). With r = as^2 + bs^2 of a row (a, b),
+// s = r + c^2, so
+//   step A: Q[c][r] = sum over the rows of class r (in sorted order) of w(c)|f|^2 --
+//           whole c-rows are read, coalesced; RB consecutive r of one block are
+//           summed into an LDS tile and written out transposed;
+//   step B: P(s) = sum_{c ascending} Q[c][s - c^2], coalesced over consecutive s.
+// Fixed summation orders, no atomics: bit-reproducible.
+constexpr int BIN_LDS = 4096;  // doubles of the step-A tile (32 KB: several blocks per CU)
+constexpr int BIN_CQ = 9;      // 64-column chunks of a row: H = N/2 + 1 <= 513
+
+template <bool MODE>
+__global__ __launch_bounds__(256) void k_bin_rows(const double2 *__restrict__ fk, const double2 *__restrict__ dk,
+                                                  const int32_t *__restrict__ roff, const int32_t *__restrict__ rows,
+                                                  int64_t H, int64_t nr, int lrb, double m2, double kir2, double a2,
+                                                  double *__restrict__ Q) {
+  __shared__ double tile[BIN_LDS];
+  const int RB = 1 << lrb;
+  const int64_t r0 = (int64_t)blockIdx.x * RB;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int rl = wave; rl < RB; rl += 4) {  // each wavefront its own row classes
+    const int64_t r = r0 + rl;
+    if (r >= nr) break;
+    const int32_t beg = roff[r], end = roff[r + 1];
+    // lane holds columns c = lane + 64 q; a whole row is loaded at once (independent
+    // loads in flight), rows of the class summed in order
+    double acc[BIN_CQ];
+    double denom[BIN_CQ];
+#pragma unroll
+    for (int q = 0; q < BIN_CQ; ++q) {
+      acc[q] = 0.0;
+      const int64_t c = lane + 64 * q;
+      // fdm3d.cpp:64: m*m + (2*pi/L)*(2*pi/L)*s/(a_t*a_t)
+      denom[q] = MODE ? m2 + (kir2 * (double)(r + c * c)) / a2 : 1.0;
+    }
+    for (int32_t j = beg; j < end; ++j) {
+      const int64_t base = (int64_t)rows[j] * H;
+#pragma unroll
+      for (int q = 0; q < BIN_CQ; ++q) {
+        const int64_t c = lane + 64 * q;
+        if (c < H) {
+          const double2 u = fk[base + c];
+          double p = u.x * u.x + u.y * u.y;
+          if constexpr (MODE) {
+            const double2 w = dk[base + c];
+            p = p + (w.x * w.x + w.y * w.y) / denom[q];
+          }
+          acc[q] += (c == 0 || c == H - 1) ? p : 2.0 * p;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < BIN_CQ; ++q) {
+      const int64_t c = lane + 64 * q;
+      if (c < H) tile[rl * H + c] = acc[q];
+    }
+  }
+  __syncthreads();
+  const int64_t nrb = (r0 + RB <= nr) ? RB : nr - r0;
+  for (int64_t q = threadIdx.x; q < (int64_t)RB * H; q += blockDim.x) {
+    const int64_t c = q >> lrb, rl = q & (RB - 1);
+    if (rl < nrb) Q[c * nr + r0 + rl] = tile[rl * H + c];
+  }
 }
 
-__global__ __launch_bounds__(256) void k_bin_power(const double2 *__restrict__ fk,
-                                                   const int64_t *__restrict__ off,
-                                                   const uint32_t *__restrict__ idx, int64_t H,
-                                                   int64_t nb, double *__restrict__ bins) {
-  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
+__global__ __launch_bounds__(256) void k_bin_cols(const double *__restrict__ Q, int64_t H, int64_t nr, int64_t nb,
+                                                  double *__restrict__ bins) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nb) return;
-  const int64_t beg = off[s], end = off[s + 1];
   double acc = 0.0;
-  for (int64_t j = beg + lane; j < end; j += 64) {
-    const uint32_t e = idx[j];
-    const int64_t c = (int64_t)e % H;
-    const double2 v = fk[e];
-    const double p = v.x * v.x + v.y * v.y;
-    acc += (c == 0 || c == H - 1) ? p : 2.0 * p;
+  for (int64_t c = 0; c < H && c * c <= s; ++c) {
+    const int64_t r = s - c * c;
+    if (r < nr) acc += Q[c * nr + r];
   }
-  acc = wave_sum(acc);
-  if (lane == 0) bins[s] = acc;
-}
-
-__global__ __launch_bounds__(256) void k_bin_mode_power(const double2 *__restrict__ pk,
-                                                        const double2 *__restrict__ dk,
-                                                        const int64_t *__restrict__ off,
-                                                        const uint32_t *__restrict__ idx, int64_t H,
-                                                        int64_t nb, double m2, double kir2, double a2,
-                                                        double *__restrict__ bins) {
-  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (s >= nb) return;
-  // fdm3d.cpp:64: m*m + (2*pi/L)*(2*pi/L)*s/(a_t*a_t)
-  const double denom = m2 + (kir2 * (double)s) / a2;
-  const int64_t beg = off[s], end = off[s + 1];
-  double acc = 0.0;
-  for (int64_t j = beg + lane; j < end; j += 64) {
-    const uint32_t e = idx[j];
-    const int64_t c = (int64_t)e % H;
-    const double2 u = pk[e], w = dk[e];
-    const double p = (u.x * u.x + u.y * u.y) + (w.x * w.x + w.y * w.y) / denom;
-    acc += (c == 0 || c == H - 1) ? p : 2.0 * p;
-  }
-  acc = wave_sum(acc);
-  if (lane == 0) bins[s] = acc;
+  bins[s] = acc;
 }
 
 // radial[j] = sum_{s : isqrt(s) = j} sbins[s], j < N/2, summed in s order.
@@ -271,7 +299,7 @@ cf_status cf_spectrum_plan_destroy(cf_spectrum_plan *p) {
   if (!p) return CF_OK;
   if (p->have_d2z) hipfftDestroy(p->d2z);
   if (p->have_z2d) hipfftDestroy(p->z2d);
-  void *bufs[] = {p->work, p->rbuf, p->cbuf0, p->cbuf1, p->bin_off, p->bin_idx, p->red};
+  void *bufs[] = {p->work, p->rbuf, p->cbuf0, p->cbuf1, p->roff, p->rows, p->red};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   delete p;
@@ -287,6 +315,7 @@ cf_status cf_spectrum_plan_create(int64_t N, cf_spectrum_plan **out) {
   p->H = N / 2 + 1;
   p->nb = cf_spectrum_bins(N);
   p->nhalf = N * N * p->H;
+  p->nr = 2 * (N / 2) * (N / 2) + 1;
   auto fail = [&](cf_status s) {
     cf_spectrum_plan_destroy(p);
     return s;
@@ -310,8 +339,8 @@ cf_status cf_spectrum_plan_create(int64_t N, cf_spectrum_plan **out) {
       hipMalloc((void **)&p->rbuf, n3 * sizeof(double)) != hipSuccess ||
       hipMalloc((void **)&p->cbuf0, 2 * (size_t)p->nhalf * sizeof(double)) != hipSuccess ||
       hipMalloc((void **)&p->cbuf1, 2 * (size_t)p->nhalf * sizeof(double)) != hipSuccess ||
-      hipMalloc((void **)&p->bin_off, ((size_t)p->nb + 1) * sizeof(int64_t)) != hipSuccess ||
-      hipMalloc((void **)&p->bin_idx, (size_t)p->nhalf * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void **)&p->roff, ((size_t)p->nr + 1) * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc((void **)&p->rows, (size_t)(N * N) * sizeof(int32_t)) != hipSuccess ||
       hipMalloc((void **)&p->red, (RED_BLOCKS + 1) * sizeof(double)) != hipSuccess) {
     set_error("cf_spectrum_plan_create(N=%lld): device allocation failed", (long long)N);
     return fail(CF_E_NOMEM);
@@ -323,31 +352,21 @@ cf_status cf_spectrum_plan_create(int64_t N, cf_spectrum_plan **out) {
       return fail(CF_E_FFT);
     }
   }
-  // Bin table: stable counting sort of half-spectrum entries by s.
+  // Row table: stable counting sort of the N^2 rows (a, b) by r = as^2 + bs^2 (the
+  // binning's step A; its output Q, H * nr <= N^3 doubles, lives in rbuf).
   {
-    const int64_t H = p->H, nb = p->nb;
-    std::vector<int64_t> off((size_t)nb + 1, 0);
-    std::vector<uint32_t> sq((size_t)N);
-    for (int64_t a = 0; a < N; ++a) {
-      const int64_t as = (a <= N / 2) ? a : N - a;
-      sq[a] = (uint32_t)(as * as);
-    }
+    const int64_t nr = p->nr;
+    std::vector<int32_t> off((size_t)nr + 1, 0);
+    auto fold = [&](int64_t v) { return (v <= N / 2) ? v : N - v; };
     for (int64_t a = 0; a < N; ++a)
-      for (int64_t b = 0; b < N; ++b) {
-        const int64_t r = sq[a] + sq[b];
-        for (int64_t c = 0; c < H; ++c) off[r + c * c + 1]++;
-      }
-    for (int64_t s = 0; s < nb; ++s) off[s + 1] += off[s];
-    std::vector<int64_t> pos(off.begin(), off.end() - 1);
-    std::vector<uint32_t> idx((size_t)p->nhalf);
-    uint32_t e = 0;
+      for (int64_t b = 0; b < N; ++b) off[fold(a) * fold(a) + fold(b) * fold(b) + 1]++;
+    for (int64_t q = 0; q < nr; ++q) off[q + 1] += off[q];
+    std::vector<int32_t> pos(off.begin(), off.end() - 1);
+    std::vector<int32_t> rws((size_t)(N * N));
     for (int64_t a = 0; a < N; ++a)
-      for (int64_t b = 0; b < N; ++b) {
-        const int64_t r = sq[a] + sq[b];
-        for (int64_t c = 0; c < H; ++c, ++e) idx[pos[r + c * c]++] = e;
-      }
-    if (hipMemcpy(p->bin_off, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->bin_idx, idx.data(), idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+      for (int64_t b = 0; b < N; ++b) rws[pos[fold(a) * fold(a) + fold(b) * fold(b)]++] = (int32_t)(a * N + b);
+    if (hipMemcpy(p->roff, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->rows, rws.data(), rws.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
       set_error("cf_spectrum_plan_create: bin table upload failed");
       return fail(CF_E_HIP);
     }
@@ -370,13 +389,29 @@ cf_status cf_fft_z2d(cf_spectrum_plan *p, double *in, double *out, cf_stream str
   return z2d_exec(p, in, out, S(stream));
 }
 
+// Step A into the plan's real scratch (free once the transform has run), step B into bins.
+static cf_status bin_launch(cf_spectrum_plan *p, const double *fk, const double *dk, double m2, double kir2,
+                            double a2, double *bins, hipStream_t st) {
+  CF_REQUIRE(p->H <= 64 * BIN_CQ, "binning: N=%lld exceeds the row chunking", (long long)p->N);
+  int lrb = 0;  // RB = 2^lrb row classes per block: the largest power of two with RB * H <= BIN_LDS
+  while ((2 << lrb) * p->H <= BIN_LDS && lrb < 8) ++lrb;
+  const dim3 gA((unsigned)cdiv(p->nr, (int64_t)1 << lrb)), b256(256);
+  if (dk)
+    hipLaunchKernelGGL(k_bin_rows<true>, gA, b256, 0, st, (const double2 *)fk, (const double2 *)dk, p->roff, p->rows,
+                       p->H, p->nr, lrb, m2, kir2, a2, p->rbuf);
+  else
+    hipLaunchKernelGGL(k_bin_rows<false>, gA, b256, 0, st, (const double2 *)fk, (const double2 *)nullptr, p->roff,
+                       p->rows, p->H, p->nr, lrb, m2, kir2, a2, p->rbuf);
+  CF_LAUNCHED();
+  hipLaunchKernelGGL(k_bin_cols, dim3((unsigned)cdiv(p->nb, 256)), b256, 0, st, p->rbuf, p->H, p->nr, p->nb, bins);
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
 cf_status cf_bin_power(cf_spectrum_plan *p, const double *fk, double *bins, cf_stream stream) {
   CF_TRY(plan_check(p));
   CF_REQUIRE(fk && bins, "cf_bin_power: null buffer");
-  hipLaunchKernelGGL(k_bin_power, dim3((unsigned)cdiv(p->nb, 4)), dim3(256), 0, S(stream),
-                     (const double2 *)fk, p->bin_off, p->bin_idx, p->H, p->nb, bins);
-  CF_LAUNCHED();
-  return CF_OK;
+  return bin_launch(p, fk, nullptr, 0.0, 0.0, 1.0, bins, S(stream));
 }
 
 cf_status cf_bin_mode_power(cf_spectrum_plan *p, const double *phik, const double *dphik, double L,
@@ -385,11 +420,7 @@ cf_status cf_bin_mode_power(cf_spectrum_plan *p, const double *phik, const doubl
   CF_REQUIRE(phik && dphik && bins, "cf_bin_mode_power: null buffer");
   const double pi = 3.14159265358979323846;
   const double kir2 = (2 * pi / L) * (2 * pi / L);
-  hipLaunchKernelGGL(k_bin_mode_power, dim3((unsigned)cdiv(p->nb, 4)), dim3(256), 0, S(stream),
-                     (const double2 *)phik, (const double2 *)dphik, p->bin_off, p->bin_idx, p->H, p->nb,
-                     m * m, kir2, a_t * a_t, bins);
-  CF_LAUNCHED();
-  return CF_OK;
+  return bin_launch(p, phik, dphik, m * m, kir2, a_t * a_t, bins, S(stream));
 }
 
 cf_status cf_power_spectrum(cf_spectrum_plan *p, const double *f, double *bins, cf_stream stream) {
