@@ -155,7 +155,7 @@ def test_split_join(cfl):
     assert rel < 2.0 ** -46
 
 
-@pytest.mark.parametrize("N", [8, 16, 64, 96])
+@pytest.mark.parametrize("N", [4, 6, 8, 16, 64, 96])
 def test_power_spectrum(cfl, O, N):
     print(f"case power spectrum N={N}", flush=True)
     f = np.random.default_rng(7).standard_normal(N ** 3)
@@ -168,7 +168,7 @@ def test_power_spectrum(cfl, O, N):
     assert_bitwise(host(plan.power_spectrum(fd)), g, "repeat")
 
 
-@pytest.mark.parametrize("N", [8, 64, 96])
+@pytest.mark.parametrize("N", [6, 8, 64, 96])
 def test_mode_power_spectrum(cfl, O, N):
     print(f"case mode spectrum N={N}", flush=True)
     x = rand_state(N, 8)

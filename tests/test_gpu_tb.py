@@ -13,7 +13,8 @@ pytestmark = pytest.mark.gpu
 FRW = dict(a1=1.0, H1=0.5, t1=1.0)
 
 
-@pytest.mark.parametrize("N,lam,steps,chunk", [(8, 0.0, 3, 0), (16, 0.1, 2, 0), (24, 0.0, 1, 0),
+@pytest.mark.parametrize("N,lam,steps,chunk", [(4, 0.1, 2, 0), (6, 0.0, 3, 0), (8, 0.0, 3, 0), (16, 0.1, 2, 0),
+                                               (24, 0.0, 1, 0),
                                                (40, 0.1, 3, 7), (64, 0.0, 2, 16), (96, 0.1, 3, 0)])
 def test_tb_f64_bitwise_oracle(cfl, O, N, lam, steps, chunk):
     print(f"case tb f64 N={N} lam={lam} steps={steps} chunk={chunk}", flush=True)
@@ -28,7 +29,7 @@ def test_tb_f64_bitwise_oracle(cfl, O, N, lam, steps, chunk):
     assert_bitwise(host(xd), x, "tb state")
 
 
-@pytest.mark.parametrize("N,lam,steps", [(16, 0.0, 3), (64, 0.1, 2), (96, 0.0, 1)])
+@pytest.mark.parametrize("N,lam,steps", [(4, 0.0, 2), (6, 0.1, 1), (16, 0.0, 3), (64, 0.1, 2), (96, 0.0, 1)])
 def test_tb_f32c_bitwise_oracle(cfl, O, N, lam, steps):
     print(f"case tb f32c N={N} lam={lam} steps={steps}", flush=True)
     L = float(N)
