@@ -492,9 +492,15 @@ def main():
                      "algorithmic_bytes_per_site_step": canon,
                      "kernel_bytes_per_site_step": DESIGN_BYTES_PER_SITE_STEP[head_mode],
                      "traffic": traffic,
+                     "traffic_GBs": traffic / (head["ms_per_step"] * 1e-3) / 1e9 if traffic else None,
+                     "traffic_frac": traffic / (head["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic
+                     else None,
                      "note": "achieved = canonical per-stage-schedule bytes x sites / launch time; a "
                              "temporally blocked step moves fewer bytes (kernel_bytes_per_site_step; "
-                             "traffic = rocprofv3 FETCH_SIZE+WRITE_SIZE per launch, calibrated)"},
+                             "traffic = rocprofv3 FETCH_SIZE+WRITE_SIZE per launch, calibrated; "
+                             "traffic_GBs = traffic / launch time). The temporally blocked kernel is "
+                             "bound by vector-ALU issue, not HBM (VALU busy ~70-76%, "
+                             "profiles/rk4_tb_N512_pmc_valu.md)"},
     }
     for a, b in (("tb_f32c", "tb_f64"), ("f32c", "f64"), ("tb_f32c", "f64")):
         if a in results and b in results:
