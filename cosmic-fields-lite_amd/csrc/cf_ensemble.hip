@@ -122,14 +122,25 @@ __global__ __launch_bounds__(1024) void k_ens_tb(const EnsReal *__restrict__ rea
 }
 
 // Flat form: one block per unit (units sorted longest first, so the hardware
-// dispatcher balances the tail), the body of the single-lattice kernel.
+// dispatcher balances the tail), the body of the single-lattice kernel. Blocks go
+// to the 8 XCDs in turn; groups of ENS_XCD_GROUP consecutive units (neighbouring
+// tiles of one lattice, whose halo rows overlap) are dealt to one XCD, so those rows
+// meet in one L2, while the long-first order stays balanced across XCDs.
+constexpr int ENS_XCD_GROUP = 4;
+
 template <class T, bool F32C, bool LAM>
 __global__ __launch_bounds__(1024) void k_ens_tb_flat(const EnsReal *__restrict__ reals,
                                                       const EnsUnit *__restrict__ units,
                                                       const EnsStepCoef *__restrict__ steps, int64_t step,
                                                       int parity, double c_m, double lam, double dt) {
   __shared__ TbLds<T> lds;
-  ens_unit<T, F32C, LAM, true, !F32C>(lds, reals, units[blockIdx.x], steps[step], parity, c_m, lam, dt);
+  int u = blockIdx.x;
+  const int total = gridDim.x;
+  if (total % (8 * ENS_XCD_GROUP) == 0) {  // a bijection of [0, total); otherwise dispatch order
+    const int x = u & 7, i = u >> 3;
+    u = ((i / ENS_XCD_GROUP) * 8 + x) * ENS_XCD_GROUP + (i % ENS_XCD_GROUP);
+  }
+  ens_unit<T, F32C, LAM, true, !F32C>(lds, reals, units[u], steps[step], parity, c_m, lam, dt);
 }
 
 }  // namespace cf
