@@ -128,7 +128,9 @@ cf_status cf_rk4_run_f32c(const cf_kg_params *p, float *hi_dev, float *lo_dev, d
  * once and written once per step, 32 B/site-step), same arithmetic and bits as
  * cf_rk4_run_f64 / cf_rk4_run_f32c. Steps ping-pong between the state and
  * work_dev (cf_rk4_tb_work_bytes); the result is always left in the state.
- * chunk = planes per block along a (<= 0: default). */
+ * chunk = planes per block along a (<= 0: default). FP32-comp speed depends on the
+ * relative placement of hi, lo and work: allocate them separately, or at relative
+ * offsets that are multiples of 4 KB. */
 size_t cf_rk4_tb_work_bytes(int64_t N, int precision);
 cf_status cf_rk4_run_tb_f64(const cf_kg_params *p, double *x_dev, double t0, int64_t k0, int64_t n_steps,
                             double dt, int chunk, void *work_dev, cf_stream stream);
