@@ -9,6 +9,8 @@
 // state), so runs ping-pong between two state arrays.
 #include <cmath>
 
+#include <vector>
+
 #include "cf_tb_body.hpp"
 #include "cf_tb_run.hpp"
 
@@ -30,8 +32,7 @@ __global__ __launch_bounds__(1024) void k_rk4_tb(TbGeom g, TbStep k, const T *__
   __shared__ TbLds<T> lds;
   // Blocks go to the 8 XCDs round robin; consecutive logical tiles are mapped
   // to one XCD so that neighbouring tiles' halos meet in its L2.
-  int bid = blockIdx.x;
-  if ((g.total & 7) == 0) bid = (bid & 7) * (g.total >> 3) + (bid >> 3);
+  const int bid = tb_xcd_order(blockIdx.x, g.total);
   const int tc = bid % g.nt;
   const int rest = bid / g.nt;
   const int tb = rest % g.nt;
@@ -54,8 +55,7 @@ template <int VAR>
 __global__ __launch_bounds__(1024) void k_rk4_tb_var(TbGeom g, TbStep k, const double *__restrict__ xin,
                                                      double *__restrict__ xout, const double *__restrict__ psi) {
   __shared__ TbLds<double> lds;
-  int bid = blockIdx.x;
-  if ((g.total & 7) == 0) bid = (bid & 7) * (g.total >> 3) + (bid >> 3);
+  const int bid = tb_xcd_order(blockIdx.x, g.total);
   const int tc = bid % g.nt;
   const int rest = bid / g.nt;
   const int tb = rest % g.nt;
@@ -81,7 +81,36 @@ static TbGeom tb_geom(int64_t N, int A) {
   return g;
 }
 
-static int tb_default_chunk(int64_t N) { return N >= 256 ? 256 : (int)N; }
+// Default planes per block along a. One 1024-thread block runs per CU at a time,
+// so k chunks per tile cost about ceil(tiles k / CUs) rounds of (N/k + 8) plane
+// iterations (8 warm-up planes per chunk). Too few blocks leave CUs idle (N=256 with
+// one chunk: 121 blocks for 256 CUs), too many pay warm-ups. Among the k within 2%
+// of the cheapest, the largest is taken. Measured on MI355X (FP64 / FP32-comp
+// GSite*step/s): N=128 chunk 19 66 / 55 against 16 / 12 with one chunk; N=256 chunk
+// 128 103 / 89 against 65 / 51; N=384 one chunk 118 / 88 against 92 / 74 with 256;
+// N=512 chunk 256 (the rule's pick) and 171 within 1-3% of each other, one chunk
+// 96 / 77.
+static int tb_default_chunk(int64_t N) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  const int64_t nt = cdiv(N, tb_tile(N)), tiles = nt * nt;
+  const int64_t kmax = N / 8 > 1 ? N / 8 : 1;
+  std::vector<int64_t> cost((size_t)kmax + 1, -1);
+  int64_t best = -1;
+  for (int64_t k = 1; k <= kmax; ++k) {
+    const int64_t ch = cdiv(N, k);
+    const int64_t kk = cdiv(N, ch);  // chunks actually launched for this chunk length
+    if (kk != k) continue;
+    cost[k] = cdiv(tiles * k, cus) * (ch + 8);
+    if (best < 0 || cost[k] < best) best = cost[k];
+  }
+  int64_t pick = 1;
+  for (int64_t k = 1; k <= kmax; ++k)
+    if (cost[k] >= 0 && cost[k] * 100 <= best * 102) pick = k;
+  return (int)cdiv(N, pick);
+}
 
 template <class T, bool F32C>
 static cf_status tb_launch(const TbGeom &g, const TbStep &k, const T *xin, const float *lin, T *xout,

@@ -86,6 +86,15 @@ __device__ __forceinline__ double tb_force(double phi, double pa1, double pam, d
   }
 }
 
+// Logical tile of hardware block b of a grid of `total` blocks. The dispatcher deals
+// blocks to the 8 XCDs in turn (b -> XCD b % 8, its (b / 8)-th block); XCD x is given
+// the contiguous logical range [x q + min(x, r), ...) of q or q + 1 tiles (q, r =
+// total / 8, total % 8), so neighbouring tiles share one L2. A bijection for any total.
+__device__ __forceinline__ int tb_xcd_order(int b, int total) {
+  const int x = b & 7, i = b >> 3, q = total >> 3, r = total & 7;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // float rounding of the compensated mode (identity for FP64)
 template <bool F32C>
 __device__ __forceinline__ double rnd(double v) {
