@@ -37,7 +37,7 @@ class AdjointProblem:
         self.weights = torch.from_numpy(w).cuda()
         nbytes = lib.cf_adjoint_work_bytes(self.N, self.n_steps, self.ckpt)
         self.work = torch.empty(int(nbytes), dtype=torch.uint8, device="cuda")
-        self.rk4_work = torch.empty(int(lib.cf_rk4_work_bytes(self.N, 0)), dtype=torch.uint8, device="cuda")
+        self.rk4_work = torch.empty(int(lib.cf_rk4_tb_work_bytes(self.N, 0)), dtype=torch.uint8, device="cuda")
 
     def params(self, m):
         return kg_params(self.N, self.L, float(m), self.lam, self.cosmo)
@@ -46,11 +46,11 @@ class AdjointProblem:
         return torch.cat([phi0.reshape(-1).to(torch.float64), torch.zeros_like(phi0.reshape(-1), dtype=torch.float64)])
 
     def loss(self, phi0, m):
-        """Forward only: L(phi0, m)."""
+        """Forward only: L(phi0, m) (the temporally blocked step: the adjoint's forward bits)."""
         x = self.state0(phi0).contiguous()
         p = self.params(m)
-        check(lib.cf_rk4_run_f64(C.byref(p), _ptr(x), self.t0, 0, self.n_steps, self.dt, _ptr(self.rk4_work),
-                                 _stream()))
+        check(lib.cf_rk4_run_tb_f64(C.byref(p), _ptr(x), self.t0, 0, self.n_steps, self.dt, 0, _ptr(self.rk4_work),
+                                    _stream()))
         return self.loss_at(x, m)
 
     def loss_at(self, x, m):

@@ -212,9 +212,6 @@ using namespace cf;
 
 extern "C" {
 
-cf_status cf_rk4_run_f64(const cf_kg_params *p, double *x, double t0, int64_t k0, int64_t n_steps, double dt,
-                         void *work, cf_stream stream);
-
 size_t cf_adjoint_work_bytes(int64_t N, int64_t n_steps, int64_t ckpt) {
   if (!valid_N(N) || n_steps < 0 || ckpt <= 0) return 0;
   const size_t v = 2 * (size_t)N * N * N * sizeof(double);
@@ -362,7 +359,7 @@ cf_status cf_adjoint_spectrum_loss(cf_spectrum_plan *plan, const cf_kg_params *p
   for (int64_t c = 0; c < nck; ++c) {
     CF_HIP(hipMemcpyAsync(ck + (size_t)c * n, x, vb, hipMemcpyDeviceToDevice, st));
     const int64_t k0 = c * ckpt, k1 = (k0 + ckpt < n_steps) ? k0 + ckpt : n_steps;
-    CF_TRY(cf_rk4_run_f64(p, x, t0, k0, k1 - k0, dt, fw, stream));
+    CF_TRY(cf_rk4_run_tb_f64(p, x, t0, k0, k1 - k0, dt, 0, fw, stream));  // same bits as the stage kernels
   }
   const double t_end = t0 + static_cast<double>(n_steps) * dt;
   double dm_loss = 0.0;
@@ -373,9 +370,9 @@ cf_status cf_adjoint_spectrum_loss(cf_spectrum_plan *plan, const cf_kg_params *p
     const int64_t k0 = c * ckpt, k1 = (k0 + ckpt < n_steps) ? k0 + ckpt : n_steps;
     CF_HIP(hipMemcpyAsync(seg, ck + (size_t)c * n, vb, hipMemcpyDeviceToDevice, st));
     for (int64_t k = k0 + 1; k < k1; ++k) {
-      double *prev = seg + (size_t)(k - 1 - k0) * n, *cur = seg + (size_t)(k - k0) * n;
-      CF_HIP(hipMemcpyAsync(cur, prev, vb, hipMemcpyDeviceToDevice, st));
-      CF_TRY(cf_rk4_run_f64(p, cur, t0, k - 1, 1, dt, fw, stream));
+      const double *prev = seg + (size_t)(k - 1 - k0) * n;
+      double *cur = seg + (size_t)(k - k0) * n;
+      CF_TRY(cf_rk4_step_tb_f64(p, prev, cur, t0 + static_cast<double>(k - 1) * dt, dt, 0, stream));
     }
     for (int64_t k = k1 - 1; k >= k0; --k) {
       const double t = t0 + static_cast<double>(k) * dt;
