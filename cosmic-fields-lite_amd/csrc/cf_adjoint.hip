@@ -99,8 +99,10 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_y(Tiling tl, RhsCoef rk, co
   }
 }
 
-// One backward stage: a = ck*lp + dk*yb_prev (HAS_YB) or ck*lp; out = J(y)^T a;
-// MODE 0: acc = lp + out, ybar = out; MODE 1: acc += out, ybar = out; MODE 2: acc += out.
+// One backward stage: a = ck*lp + dk*yb_prev (HAS_YB) or ck*lp; out = J(y)^T a.
+// MODE 0 and 1: ybar = out (each stage's output kept in its own buffer); MODE 2 (the
+// last): acc = (((lp + yb4) + yb3) + yb_prev) + out, the lambda of the step in the
+// order of a running sum, without reading and rewriting an accumulator per stage.
 // part[blockIdx] = sum over the block's sites of (-2 m phi_y) a_pi, in a fixed order.
 // A fixed grid of ADJ_BLOCKS blocks walks the tiles (one thread per (b,c) column
 // marching along a, a_pi at a-1, a, a+1 kept in registers).
@@ -108,7 +110,8 @@ template <int MODE, bool HAS_YB>
 __global__ __launch_bounds__(ADJ_THREADS) void k_adj_back(Tiling tl, const double *__restrict__ lp,
                                                           const double *__restrict__ ybp,
                                                           const double *__restrict__ y, double *__restrict__ ybar,
-                                                          double *__restrict__ acc, AdjCoef k,
+                                                          double *__restrict__ acc, const double *__restrict__ yb4,
+                                                          const double *__restrict__ yb3, AdjCoef k,
                                                           double *__restrict__ part) {
   const int64_t N = tl.N, NN = N * N, N3 = NN * N;
   auto api = [&](int64_t j) {
@@ -141,14 +144,10 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_back(Tiling tl, const doubl
       const double phi = y[i];
       const double out_phi = (-k.c_m - 3.0 * k.lam * phi * phi) * a0v + k.inv * (nb - 6.0 * a0v);
       const double out_pi = aphi + k.c_h * a0v;
-      if constexpr (MODE == 0) {
-        acc[i] = lp[i] + out_phi;
-        acc[N3 + i] = lp[N3 + i] + out_pi;
+      if constexpr (MODE == 2) {
+        acc[i] = (((lp[i] + yb4[i]) + yb3[i]) + ybp[i]) + out_phi;
+        acc[N3 + i] = (((lp[N3 + i] + yb4[N3 + i]) + yb3[N3 + i]) + ybp[N3 + i]) + out_pi;
       } else {
-        acc[i] += out_phi;
-        acc[N3 + i] += out_pi;
-      }
-      if constexpr (MODE != 2) {
         ybar[i] = out_phi;
         ybar[N3 + i] = out_pi;
       }
@@ -283,7 +282,7 @@ cf_status cf_rk4_step_adjoint_f64(const cf_kg_params *p, const double *x, double
   const int64_t N = p->N, n = 2 * N * N * N;
   hipStream_t st = S(stream);
   double *y1 = (double *)work_dev, *y2 = y1 + n, *y3 = y2 + n;
-  double *yb0 = y3 + 2 * n, *yb1 = yb0 + n, *acc = yb1 + n;  // (y3 + n: unused)
+  double *yb4 = y3 + n, *yb3 = yb4 + n, *yb2 = yb3 + n, *acc = yb2 + n;
   const double dh = dt / 2.0, th = t + dh, dt6 = dt / 6.0, dt3 = dt / 3.0;
   const Tiling tl = make_tiling(N, default_chunk(N));
   const dim3 blk(TC, TB);
@@ -319,13 +318,14 @@ cf_status cf_rk4_step_adjoint_f64(const cf_kg_params *p, const double *x, double
     return k;
   };
   const dim3 g(ADJ_BLOCKS);
-  hipLaunchKernelGGL((k_adj_back<0, false>), g, blk, 0, st, tl, lam, (const double *)nullptr, y3, yb0, acc,
+  const double *no = nullptr;
+  hipLaunchKernelGGL((k_adj_back<0, false>), g, blk, 0, st, tl, lam, no, y3, yb4, (double *)nullptr, no, no,
                      coef(t + dt, dt6, 0.0), part_dev);
-  hipLaunchKernelGGL((k_adj_back<1, true>), g, blk, 0, st, tl, lam, yb0, y2, yb1, acc, coef(th, dt3, dt),
-                     part_dev + ADJ_BLOCKS);
-  hipLaunchKernelGGL((k_adj_back<1, true>), g, blk, 0, st, tl, lam, yb1, y1, yb0, acc, coef(th, dt3, dh),
-                     part_dev + 2 * ADJ_BLOCKS);
-  hipLaunchKernelGGL((k_adj_back<2, true>), g, blk, 0, st, tl, lam, yb0, x, (double *)nullptr, acc,
+  hipLaunchKernelGGL((k_adj_back<1, true>), g, blk, 0, st, tl, lam, yb4, y2, yb3, (double *)nullptr, no, no,
+                     coef(th, dt3, dt), part_dev + ADJ_BLOCKS);
+  hipLaunchKernelGGL((k_adj_back<1, true>), g, blk, 0, st, tl, lam, yb3, y1, yb2, (double *)nullptr, no, no,
+                     coef(th, dt3, dh), part_dev + 2 * ADJ_BLOCKS);
+  hipLaunchKernelGGL((k_adj_back<2, true>), g, blk, 0, st, tl, lam, yb2, x, (double *)nullptr, acc, yb4, yb3,
                      coef(t, dt6, dh), part_dev + 3 * ADJ_BLOCKS);
   CF_LAUNCHED();
   CF_HIP(hipMemcpyAsync(lam, acc, n * sizeof(double), hipMemcpyDeviceToDevice, st));
