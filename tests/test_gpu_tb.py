@@ -1,7 +1,8 @@
 """Temporally blocked RK4 (one kernel per step) against the oracle and the
 four-kernel path: bit-identical states for every lattice size class (N smaller
 than the tile, partial edge tiles, several a-chunks, odd step counts that end
-with the ping-pong copy) and both precisions."""
+with the ping-pong copy, the default chunk of the launch cost model -- N=128: 19
+planes, a partial last chunk) and both precisions."""
 import numpy as np
 import pytest
 import torch
@@ -14,7 +15,7 @@ FRW = dict(a1=1.0, H1=0.5, t1=1.0)
 
 
 @pytest.mark.parametrize("N,lam,steps,chunk", [(4, 0.1, 2, 0), (6, 0.0, 3, 0), (8, 0.0, 3, 0), (16, 0.1, 2, 0),
-                                               (24, 0.0, 1, 0),
+                                               (24, 0.0, 1, 0), (128, 0.1, 2, 0),
                                                (40, 0.1, 3, 7), (64, 0.0, 2, 16), (96, 0.1, 3, 0)])
 def test_tb_f64_bitwise_oracle(cfl, O, N, lam, steps, chunk):
     print(f"case tb f64 N={N} lam={lam} steps={steps} chunk={chunk}", flush=True)
@@ -29,7 +30,8 @@ def test_tb_f64_bitwise_oracle(cfl, O, N, lam, steps, chunk):
     assert_bitwise(host(xd), x, "tb state")
 
 
-@pytest.mark.parametrize("N,lam,steps", [(4, 0.0, 2), (6, 0.1, 1), (16, 0.0, 3), (64, 0.1, 2), (96, 0.0, 1)])
+@pytest.mark.parametrize("N,lam,steps", [(4, 0.0, 2), (6, 0.1, 1), (16, 0.0, 3), (64, 0.1, 2), (96, 0.0, 1),
+                                         (128, 0.0, 2)])
 def test_tb_f32c_bitwise_oracle(cfl, O, N, lam, steps):
     print(f"case tb f32c N={N} lam={lam} steps={steps}", flush=True)
     L = float(N)
