@@ -323,7 +323,8 @@ def adjoint_workload(args):
 def variants_workload(args):
     """FP64 RK4 steps of the fixed-curvature (Psi: a Gaussian random field, rms 0.1)
     and sqrt-potential (f_a = 0.7) equations at N=512, L=512, m=1.1, radiation FRW,
-    dt=0.01, from the headline's initial state: the four fused stage kernels per step
+    dt=0.01, from the headline's initial state, and of the comoving-curvature
+    equations at the reference main run's size: the four fused stage kernels per step
     against the temporally blocked kernel (same bits), HIP events on the launch stream."""
     import torch
     import cfl_amd as cfl
@@ -352,13 +353,40 @@ def variants_workload(args):
             out[f"{name}_{sched}"] = {"ms_per_step": ms, "gsite_steps_per_s": N ** 3 / ms / 1e6}
             del x
         v.close()
+    # the reference main run's comoving-curvature equations (N=384, L=307.2, m=1,
+    # radiation era a1=1, H1=0.05, t1=10, dt=0.5; coarse grid M=128), R a Gaussian
+    # random field of rms 0.02
+    Nc, Lc = 384, 307.2
+    pc = cfl.kg_params(Nc, Lc, 1.0, 0.0, cfl.radiation(a1=1.0, H1=0.05, t1=10.0))
+    plan = cfl.SpectrumPlan(Nc)
+    Rk = plan.d2z(plan.init_grf_state(5, cfl.CF_SPEC_GAUSS, (0.5,), Lc, 0.02)[: Nc ** 3].clone())
+    xc0 = plan.init_grf_state(6, cfl.CF_SPEC_GAUSS, (0.5,), Lc, 1.0)
+    del plan
+    for name, kind, M in (("comoving_curvature_N384", cfl.CF_VAR_COMOVING_CURVATURE, 0),
+                          ("approx_comoving_N384_M128", cfl.CF_VAR_APPROX_COMOVING_CURVATURE, 128)):
+        for sched in ("stage_kernels", "tb"):
+            v = cfl.Variant(pc, kind, Rk, M=M)
+            x = xc0.clone()
+            run = v.rk4_run if sched == "stage_kernels" else v.rk4_run_tb
+            run(x, 10.0, 0, W, 0.5)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            run(x, 10.0, W, K, 0.5)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / K
+            out[f"{name}_{sched}"] = {"ms_per_step": ms, "gsite_steps_per_s": Nc ** 3 / ms / 1e6}
+            v.close()
+            del x
     line = {
         "metric": "lattice site-steps per second, equation variants", "value": out["fixed_curvature_tb"][
             "gsite_steps_per_s"], "unit": "GSite*step/s (fixed curvature, temporally blocked)",
         "higher_is_better": True, "n_gpus": 1, "steps": K, "warmup": W, "dtype": "f64",
         "data": "synthetic (Gaussian random fields, seeds 2 and 3)",
         "config": {"workload": f"fixed-curvature and sqrt-potential FRW equations N={N}^3 L={N} m=1.1 f_a=0.7 "
-                               "dt=0.01 RK4"},
+                               "dt=0.01 RK4; comoving-curvature equations of the reference main run N=384^3 "
+                               "L=307.2 dt=0.5 (coarse grid M=128)"},
         "paths": out,
     }
     print(json.dumps(line), flush=True)

@@ -397,7 +397,7 @@ class Variant:
         check(lib.cf_variant_rk4_run(self.h, _ptr(x), t0, k0, n_steps, dt, _ptr(self._w()), _stream(stream)))
 
     def rk4_run_tb(self, x, t0, k0, n_steps, dt, chunk=0, stream=None):
-        """Temporally blocked steps (fixed curvature, sqrt potential); same bits as rk4_run."""
+        """Temporally blocked steps (every variant); same bits as rk4_run."""
         if self._tb_work is None:
             self._tb_work = alloc_tb_work(self.N, CF_PREC_F64)
         check(lib.cf_variant_rk4_run_tb(self.h, _ptr(x), t0, k0, n_steps, dt, chunk, _ptr(self._tb_work),

@@ -53,7 +53,8 @@ __global__ __launch_bounds__(1024) void k_rk4_tb(TbGeom g, TbStep k, const T *__
 // variants, FP64.
 template <int VAR>
 __global__ __launch_bounds__(1024) void k_rk4_tb_var(TbGeom g, TbStep k, const double *__restrict__ xin,
-                                                     double *__restrict__ xout, const double *__restrict__ psi) {
+                                                     double *__restrict__ xout, const double *__restrict__ psi,
+                                                     TbPsi cps) {
   __shared__ TbLds<double> lds;
   const int bid = tb_xcd_order(blockIdx.x, g.total);
   const int tc = bid % g.nt;
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(1024) void k_rk4_tb_var(TbGeom g, TbStep k, const d
   u.c0 = (int64_t)tc * g.T;
   u.a0 = (int64_t)ac * g.A;
   u.a1 = (u.a0 + g.A < g.N) ? u.a0 + g.A : g.N;
-  tb_unit<double, false, false, true, true, VAR, true>(lds, u, k, xin, nullptr, xout, nullptr, psi);
+  tb_unit<double, false, false, true, true, VAR, true>(lds, u, k, xin, nullptr, xout, nullptr, psi, cps);
 }
 
 static TbGeom tb_geom(int64_t N, int A) {
@@ -138,35 +139,29 @@ static cf_status tb_check(const cf_kg_params *p) {
 }
 
 cf_status tb_step_variant(int var, const cf_kg_params *p, double f_a, const double *psi, const double *xin,
-                          double *xout, double t, double dt, int chunk, hipStream_t st) {
+                          double *xout, double t, double dt, int chunk, hipStream_t st, const TbPsi *cps) {
   CF_TRY(tb_check(p));
-  CF_REQUIRE(var == TB_SQRT_POTENTIAL || (var == TB_FIXED_CURVATURE && psi), "tb_step_variant: bad variant");
+  const bool comoving = var == TB_COMOVING || var == TB_APPROX_COMOVING;
+  CF_REQUIRE(var == TB_SQRT_POTENTIAL || (var == TB_FIXED_CURVATURE && psi) ||
+                 (comoving && cps && cps->f[0] && cps->f[1] && cps->f[2] && cps->G > 0),
+             "tb_step_variant: bad variant");
   const size_t n = 2 * (size_t)p->N * p->N * p->N;
   CF_REQUIRE(xin && xout && disjoint(xin, xout, n * sizeof(double)),
              "temporally blocked variant step: null or overlapping state buffers");
   const TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N));
   TbStep k = tb_step_host(p, t, dt);
   k.fa2 = f_a * f_a;
+  const TbPsi cp = cps ? *cps : TbPsi{};
   if (var == TB_FIXED_CURVATURE)
-    hipLaunchKernelGGL(k_rk4_tb_var<TB_FIXED_CURVATURE>, dim3(g.total), dim3(1024), 0, st, g, k, xin, xout, psi);
+    hipLaunchKernelGGL(k_rk4_tb_var<TB_FIXED_CURVATURE>, dim3(g.total), dim3(1024), 0, st, g, k, xin, xout, psi, cp);
+  else if (var == TB_SQRT_POTENTIAL)
+    hipLaunchKernelGGL(k_rk4_tb_var<TB_SQRT_POTENTIAL>, dim3(g.total), dim3(1024), 0, st, g, k, xin, xout, psi, cp);
+  else if (var == TB_COMOVING)
+    hipLaunchKernelGGL(k_rk4_tb_var<TB_COMOVING>, dim3(g.total), dim3(1024), 0, st, g, k, xin, xout, psi, cp);
   else
-    hipLaunchKernelGGL(k_rk4_tb_var<TB_SQRT_POTENTIAL>, dim3(g.total), dim3(1024), 0, st, g, k, xin, xout, psi);
+    hipLaunchKernelGGL(k_rk4_tb_var<TB_APPROX_COMOVING>, dim3(g.total), dim3(1024), 0, st, g, k, xin, xout, psi,
+                       cp);
   CF_LAUNCHED();
-  return CF_OK;
-}
-
-cf_status tb_run_variant(int var, const cf_kg_params *p, double f_a, const double *psi, double *x, double t0,
-                         int64_t k0, int64_t n_steps, double dt, int chunk, void *work, hipStream_t st) {
-  CF_TRY(tb_check(p));
-  CF_REQUIRE(n_steps >= 0, "temporally blocked variant step: negative step count");
-  const size_t bytes = 2 * (size_t)p->N * p->N * p->N * sizeof(double);
-  CF_REQUIRE(x && work && disjoint(x, work, bytes), "temporally blocked variant step: null or overlapping buffer");
-  double *bufs[2] = {x, (double *)work};
-  for (int64_t s = 0; s < n_steps; ++s) {
-    const double t = t0 + static_cast<double>(k0 + s) * dt;
-    CF_TRY(tb_step_variant(var, p, f_a, psi, bufs[s & 1], bufs[(s + 1) & 1], t, dt, chunk, st));
-  }
-  if (n_steps & 1) CF_HIP(hipMemcpyAsync(x, work, bytes, hipMemcpyDeviceToDevice, st));
   return CF_OK;
 }
 

@@ -53,7 +53,16 @@ struct TbStep {
 // fixed-curvature / sqrt-potential variants in the operation order of their CUDA
 // kernels (Fixed_Curvature_FRW_equation_kernel, equations_cuda.cu:114-137;
 // Sqrt_Potential_FRW_equation_kernel, :83-106), as in csrc/cf_variants.hip.
-enum : int { TB_KG = 0, TB_FIXED_CURVATURE = 1, TB_SQRT_POTENTIAL = 2 };
+enum : int { TB_KG = 0, TB_FIXED_CURVATURE = 1, TB_SQRT_POTENTIAL = 2, TB_COMOVING = 3, TB_APPROX_COMOVING = 4 };
+
+// Comoving curvature: Psi | dPsi/dt ([Psi (G^3) | dPsi (G^3)]) at the step's three
+// stage times (0: t, 1: t + dt/2 for stages 2 and 3, 2: t + dt); G = N (TB_COMOVING)
+// or the coarse grid M (TB_APPROX_COMOVING, site (a,b,c) reads cell
+// ((x + hs) % N) / sd per axis, sd = N / M, hs = sd / 2).
+struct TbPsi {
+  const double *f[3];
+  int64_t G, sd, hs;
+};
 
 // One work unit: lattice size, output tile origin and edge, plane chunk.
 struct TbUnit {
@@ -73,13 +82,24 @@ __device__ __forceinline__ double tb_rhs(double phi, double pa1, double pam, dou
 
 template <int VAR, bool LAM>
 __device__ __forceinline__ double tb_force(double phi, double pa1, double pam, double pb1, double pbm, double pc1,
-                                           double pcm, double pi, double c_h, double inv, const TbStep &k, double ps) {
+                                           double pcm, double pi, double c_h, double inv, const TbStep &k, double ps,
+                                           double dp = 0.0) {
   if constexpr (VAR == TB_KG) {
     return tb_rhs(phi, pa1, pam, pb1, pbm, pc1, pcm, pi, c_h, k.c_m, k.lam, inv, LAM);
   } else if constexpr (VAR == TB_FIXED_CURVATURE) {
     const double e2 = exp(2.0 * ps), e4 = exp(4.0 * ps);
     const double s7 = (((((-6.0 * phi + pa1) + pam) + pb1) + pbm) + pc1) + pcm;
     return (c_h * pi - ((e2 * k.m) * k.m) * phi) + (e4 * inv) * s7;
+  } else if constexpr (VAR == TB_COMOVING) {
+    const double e2 = exp(2.0 * ps), e4 = exp(4.0 * ps);
+    const double ci = e4 * inv;
+    const double v = ((c_h + 4.0 * dp) * pi - ((e2 * k.m) * k.m) * phi) +
+                     ci * ((((-6.0 * phi + pa1) + pam) + pb1) + pbm);
+    return v + ci * (pc1 + pcm);
+  } else if constexpr (VAR == TB_APPROX_COMOVING) {
+    const double s7 = (((((-6.0 * phi + pa1) + pam) + pb1) + pbm) + pc1) + pcm;
+    return ((c_h + 4.0 * dp) * pi - ((((1.0 + 2.0 * ps) + (2.0 * ps) * ps) * k.m) * k.m) * phi) +
+           (((1.0 + 4.0 * ps) + (8.0 * ps) * ps) * inv) * s7;
   } else {
     const double s7 = (((((-6.0 * phi + pa1) + pam) + pb1) + pbm) + pc1) + pcm;
     return (c_h * pi - (k.c_m * phi) / sqrt(1.0 + (phi * phi) / k.fa2)) + inv * s7;
@@ -172,7 +192,8 @@ using TbLds = T[2][4][TBR][TBR];
 template <class T, bool F32C, bool LAM, bool DEFER = true, bool LANE_NBR = true, int VAR = TB_KG, bool RING = false>
 __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const TbStep &k, const T *__restrict__ xin,
                                         const float *__restrict__ lin, T *__restrict__ xout,
-                                        float *__restrict__ lout, const double *__restrict__ psi = nullptr) {
+                                        float *__restrict__ lout, const double *__restrict__ psi = nullptr,
+                                        const TbPsi cps = TbPsi{}) {
   static_assert(VAR == TB_KG || !F32C, "the equation variants are FP64");
   const int tid = threadIdx.x;
   const int r = tid >> 5, cc = tid & 31;
@@ -253,6 +274,31 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
   XP[1] = (Hv)xin[N3 + off_ld + col];
   if constexpr (VAR == TB_FIXED_CURVATURE) PS[1] = psi[off_ld + col];
   int64_t off_st = wrap_plane(u.a0 - TBH - 4);  // plane P-4
+  int64_t a_st = off_st / NN;                    // its index
+  // comoving curvature: Psi and dPsi of stage s (plane P-s) from time slot `slot`
+  int64_t cell_bc = 0;
+  if constexpr (VAR == TB_APPROX_COMOVING) {
+    const int64_t jb = ((b + cps.hs) % N) / cps.sd, jc = ((c + cps.hs) % N) / cps.sd;
+    cell_bc = jb * cps.G + jc;
+  }
+  auto cpsi = [&](int s, int slot, double &ps, double &dp) {
+    if constexpr (VAR == TB_COMOVING) {
+      int64_t pl = off_st + (int64_t)(4 - s) * NN;
+      if (pl >= N3) pl -= N3;
+      ps = cps.f[slot][pl + col];
+      dp = cps.f[slot][N3 + pl + col];
+    } else if constexpr (VAR == TB_APPROX_COMOVING) {
+      // 32-bit index arithmetic (N <= 2048)
+      const int n = (int)N, sd = (int)cps.sd, hs = (int)cps.hs;
+      int as = (int)a_st + (4 - s);
+      if (as >= n) as -= n;
+      int ah = as + hs;
+      if (ah >= n) ah -= n;
+      const int64_t j = (int64_t)(ah / sd) * (cps.G * cps.G) + cell_bc;
+      ps = cps.f[slot][j];
+      dp = cps.f[slot][cps.G * cps.G * cps.G + j];
+    }
+  };
   TbPending<T> Q[2] = {};
   Q[0].off = Q[1].off = TB_OOB;
   // F32C: the compensation term of output plane P-4 is prefetched the same way
@@ -313,11 +359,26 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     buf[3][r][cc] = (T)Y3F[I1];
     __syncthreads();
 
+    // the stages' Psi: fixed curvature from its plane history, comoving from the
+    // stage time's field at the stage's plane
+    double ps1 = 0.0, dp1 = 0.0, ps2 = 0.0, dp2 = 0.0, ps3 = 0.0, dp3 = 0.0, ps4 = 0.0, dp4 = 0.0;
+    if constexpr (VAR == TB_FIXED_CURVATURE) {
+      ps1 = PS[I1];
+      ps2 = PS[I2];
+      ps3 = PS[I3];
+      ps4 = PS[I4];
+    } else if constexpr (VAR == TB_COMOVING || VAR == TB_APPROX_COMOVING) {
+      cpsi(1, 0, ps1, dp1);
+      cpsi(2, 1, ps2, dp2);
+      cpsi(3, 1, ps3, dp3);
+      cpsi(4, 2, ps4, dp4);
+    }
+
     // ---- stage 1 at P-1: k1 = f(x)
     double kf = (double)XP[I1];
     double kp = tb_force<VAR, LAM>((double)XF[I1], (double)XF[I0], (double)XF[I2], (double)buf[0][rp][cc],
                                   (double)buf[0][rm][cc], cnext(XF[I1], buf[0]), cprev(XF[I1], buf[0]),
-                                  (double)XP[I1], k.c_h[0], k.inv[0], k, PS[I1]);
+                                  (double)XP[I1], k.c_h[0], k.inv[0], k, ps1, dp1);
     Y1F[I0] = (Hv)rnd<F32C>((double)XF[I1] + k.dh * kf);
     Y1P[I0] = (Hv)rnd<F32C>((double)XP[I1] + k.dh * kp);
     if constexpr (F32C) {
@@ -332,7 +393,7 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     kf = (double)Y1P[I1];
     kp = tb_force<VAR, LAM>((double)Y1F[I1], (double)Y1F[I0], (double)Y1F[I2], (double)buf[1][rp][cc],
                             (double)buf[1][rm][cc], cnext(Y1F[I1], buf[1]), cprev(Y1F[I1], buf[1]),
-                            (double)Y1P[I1], k.c_h[1], k.inv[1], k, PS[I2]);
+                            (double)Y1P[I1], k.c_h[1], k.inv[1], k, ps2, dp2);
     Y2F[I0] = (Hv)rnd<F32C>((double)XF[I2] + k.dh * kf);
     Y2P[I0] = (Hv)rnd<F32C>((double)XP[I2] + k.dh * kp);
     AF[I1] = (Hv)((double)AF[I1] + k.dt3 * kf);
@@ -342,7 +403,7 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     kf = (double)Y2P[I1];
     kp = tb_force<VAR, LAM>((double)Y2F[I1], (double)Y2F[I0], (double)Y2F[I2], (double)buf[2][rp][cc],
                             (double)buf[2][rm][cc], cnext(Y2F[I1], buf[2]), cprev(Y2F[I1], buf[2]),
-                            (double)Y2P[I1], k.c_h[2], k.inv[2], k, PS[I3]);
+                            (double)Y2P[I1], k.c_h[2], k.inv[2], k, ps3, dp3);
     Y3F[I0] = (Hv)rnd<F32C>((double)XF[I3] + k.dt * kf);
     Y3P[I0] = (Hv)rnd<F32C>((double)XP[I3] + k.dt * kp);
     AF[I2] = (Hv)((double)AF[I2] + k.dt3 * kf);
@@ -352,7 +413,7 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     kf = (double)Y3P[I1];
     kp = tb_force<VAR, LAM>((double)Y3F[I1], (double)Y3F[I0], (double)Y3F[I2], (double)buf[3][rp][cc],
                             (double)buf[3][rm][cc], cnext(Y3F[I1], buf[3]), cprev(Y3F[I1], buf[3]),
-                            (double)Y3P[I1], k.c_h[3], k.inv[3], k, PS[I4]);
+                            (double)Y3P[I1], k.c_h[3], k.inv[3], k, ps4, dp4);
     if constexpr (F32C) {
       const double s0 = ((double)XF[I4] + lo_f) + ((double)AF[I3] + k.dt6 * kf);
       const double s1 = ((double)XP[I4] + lo_p) + ((double)AP[I3] + k.dt6 * kp);
@@ -369,6 +430,7 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     q.plane = off_st;
     if constexpr (!DEFER) flush(q);
     off_st = next_plane(off_st);
+    a_st = (a_st + 1 == N) ? 0 : a_st + 1;
   };
   using std::integral_constant;
   if constexpr (RING) {

@@ -23,8 +23,9 @@
 // (equations.cpp:141-183), followed by one batched complex-to-real transform of
 // both fields. A classic RK4 step needs them at t, t+dt/2 and t+dt only (stages
 // 2 and 3 share a time), and the next step's first stage is usually this step's
-// last, so a two-slot cache keyed by the exact stage time removes most
-// transforms; the reference computes two transforms per right-hand side.
+// last, so a three-slot cache keyed by the exact stage time leaves two transforms
+// per step; the reference computes two transforms per right-hand side. The
+// temporally blocked step reads the three times' fields in one kernel.
 //
 // All arithmetic is FP64 and, as in the rest of the library, no multiply is
 // fused into an add (-ffp-contract=off).
@@ -256,10 +257,15 @@ struct PsiGrid {
   int64_t G = 0;
   double *R = nullptr;     // 2*G*G*(G/2+1): comoving curvature, half spectrum
   double *spec = nullptr;  // 2 half spectra: Psi^, dPsi^
-  double *slot[2] = {nullptr, nullptr};  // each 2*G^3: Psi | dPsi
-  double slot_t[2] = {0.0, 0.0};
-  bool valid[2] = {false, false};
-  int last = 1;
+  // Psi | dPsi (2*G^3 each) at up to three times, least recently used replaced: a
+  // temporally blocked step needs t, t + dt/2 and t + dt at once, and the next step
+  // starts at the previous t + dt (two new times per step either way).
+  static constexpr int NS = 3;
+  double *slot[NS] = {nullptr, nullptr, nullptr};
+  double slot_t[NS] = {0.0, 0.0, 0.0};
+  bool valid[NS] = {false, false, false};
+  uint64_t used[NS] = {0, 0, 0};
+  uint64_t clock = 0;
   hipfftHandle plan = 0;
   bool have_plan = false;
 
@@ -271,7 +277,7 @@ struct PsiGrid {
     CF_REQUIRE(G <= 1024, "comoving curvature variants support grids up to 1024^3 (got %lld)", (long long)G);
     CF_HIP(hipMalloc(&R, half() * sizeof(double)));
     CF_HIP(hipMalloc(&spec, 2 * half() * sizeof(double)));
-    for (int s = 0; s < 2; ++s) CF_HIP(hipMalloc(&slot[s], 2 * cube() * sizeof(double)));
+    for (int s = 0; s < NS; ++s) CF_HIP(hipMalloc(&slot[s], 2 * cube() * sizeof(double)));
     int n[3] = {(int)G, (int)G, (int)G};
     hipfftResult r = hipfftPlanMany(&plan, 3, n, nullptr, 1, (int)(G * G * (G / 2 + 1)), nullptr, 1, (int)cube(),
                                     HIPFFT_Z2D, 2);
@@ -287,20 +293,24 @@ struct PsiGrid {
     have_plan = false;
     (void)hipFree(R);
     (void)hipFree(spec);
-    for (int s = 0; s < 2; ++s) (void)hipFree(slot[s]);
+    for (int s = 0; s < NS; ++s) {
+      (void)hipFree(slot[s]);
+      slot[s] = nullptr;
+    }
     R = spec = nullptr;
-    slot[0] = slot[1] = nullptr;
   }
 
   // Psi(t) | dPsi(t) as 2*G^3 reals, computed on stream st unless cached.
   cf_status get(double t, double factor, double H, hipStream_t st, const double **out) {
-    for (int s = 0; s < 2; ++s)
-      if (valid[s] && slot_t[s] == t) {
-        last = s;
-        *out = slot[s];
+    for (int q = 0; q < NS; ++q)
+      if (valid[q] && slot_t[q] == t) {
+        used[q] = ++clock;
+        *out = slot[q];
         return CF_OK;
       }
-    const int s = 1 - last;
+    int s = 0;
+    for (int q = 1; q < NS; ++q)
+      if (used[q] < used[s]) s = q;
     valid[s] = false;
     hipLaunchKernelGGL(k_comoving_transfer, dim3(grid_for((int64_t)(half() / 2))), dim3(256), 0, st, G, R, spec,
                        spec + half(), factor, H);
@@ -316,7 +326,7 @@ struct PsiGrid {
     }
     slot_t[s] = t;
     valid[s] = true;
-    last = s;
+    used[s] = ++clock;
     *out = slot[s];
     return CF_OK;
   }
@@ -544,25 +554,48 @@ cf_status cf_variant_rk4_run(cf_variant v, double *x, double t0, int64_t k0, int
   return CF_OK;
 }
 
+// One temporally blocked step of any variant from x_in into x_out (the comoving
+// ones with Psi | dPsi at t, t + dt/2, t + dt from the cache, the same fields and
+// times as the stage kernels use). The kinds equal the TB_* ids (CF_VAR_* 1-4).
+static cf_status var_tb_step(cf_variant_s *v, const double *x_in, double *x_out, double t, double dt, int chunk,
+                             hipStream_t st) {
+  if (v->kind == V_FIXED || v->kind == V_SQRT)
+    return tb_step_variant(v->kind, &v->p, v->f_a, v->psi_fixed, x_in, x_out, t, dt, chunk, st, nullptr);
+  const double dh = dt / 2.0, th = t + dh;
+  const double ts[3] = {t, th, t + dt};
+  TbPsi cp{};
+  for (int q = 0; q < 3; ++q) {
+    const double *pf = nullptr;
+    CF_TRY(v->grid.get(ts[q], comoving_factor(v->p, ts[q]), cf_cosmo_H(&v->p.cosmo, ts[q]), st, &pf));
+    cp.f[q] = pf;
+  }
+  cp.G = v->grid.G;
+  cp.sd = v->kind == V_APPROX ? v->p.N / v->M : 1;
+  cp.hs = cp.sd / 2;
+  return tb_step_variant(v->kind, &v->p, v->f_a, nullptr, x_in, x_out, t, dt, chunk, st, &cp);
+}
+
 cf_status cf_variant_rk4_run_tb(cf_variant v, double *x, double t0, int64_t k0, int64_t n_steps, double dt, int chunk,
                                 void *work, cf_stream stream) {
   CF_REQUIRE(v, "cf_variant_rk4_run_tb: null handle");
-  if (v->kind != V_FIXED && v->kind != V_SQRT) {
-    set_error("cf_variant_rk4_run_tb: the comoving variants rebuild Psi at every stage time; use cf_variant_rk4_run");
-    return CF_E_UNSUPPORTED;
+  CF_REQUIRE(n_steps >= 0, "cf_variant_rk4_run_tb: negative step count");
+  const size_t bytes = 2 * (size_t)v->p.N * v->p.N * v->p.N * sizeof(double);
+  const char *xa = (const char *)x, *wa = (const char *)work;
+  CF_REQUIRE(x && work && (xa + bytes <= wa || wa + bytes <= xa), "cf_variant_rk4_run_tb: null or overlapping buffer");
+  hipStream_t st = S(stream);
+  double *bufs[2] = {x, (double *)work};
+  for (int64_t s = 0; s < n_steps; ++s) {
+    const double t = t0 + static_cast<double>(k0 + s) * dt;
+    CF_TRY(var_tb_step(v, bufs[s & 1], bufs[(s + 1) & 1], t, dt, chunk, st));
   }
-  // TB_FIXED_CURVATURE = 1 = CF_VAR_FIXED_CURVATURE, TB_SQRT_POTENTIAL = 2 = CF_VAR_SQRT_POTENTIAL
-  return tb_run_variant(v->kind, &v->p, v->f_a, v->psi_fixed, x, t0, k0, n_steps, dt, chunk, work, S(stream));
+  if (n_steps & 1) CF_HIP(hipMemcpyAsync(x, work, bytes, hipMemcpyDeviceToDevice, st));
+  return CF_OK;
 }
 
 cf_status cf_variant_rk4_step_tb(cf_variant v, const double *x_in, double *x_out, double t, double dt, int chunk,
                                  cf_stream stream) {
   CF_REQUIRE(v, "cf_variant_rk4_step_tb: null handle");
-  if (v->kind != V_FIXED && v->kind != V_SQRT) {
-    set_error("cf_variant_rk4_step_tb: the comoving variants rebuild Psi at every stage time; use cf_variant_rk4_step");
-    return CF_E_UNSUPPORTED;
-  }
-  return tb_step_variant(v->kind, &v->p, v->f_a, v->psi_fixed, x_in, x_out, t, dt, chunk, S(stream));
+  return var_tb_step(v, x_in, x_out, t, dt, chunk, S(stream));
 }
 
 cf_status cf_variant_psi(cf_variant v, double t, double *psi, double *dpsi, cf_stream stream) {

@@ -8,11 +8,10 @@
 // so each is a drop-in beside CudaKleinGordonEquationInFRW. operator() is one
 // HIP launch (cf_rhs_f64); the odeint ops are hip_operations. Each equation also
 // offers rk4_step(x, t, dt): the whole classic RK4 step as four fused stage
-// kernels (cf_rk4_step_f64), bit-identical to sys() + scale_sum2/scale_sum5; and,
-// where the temporally blocked kernel covers the equation, rk4_step_into(x, out,
-// t, dt): the same step with the same bits in one kernel that reads x once and
-// writes out once. hip_runge_kutta4 (stepper.hpp) calls rk4_step_into when it
-// exists, else rk4_step.
+// kernels (cf_rk4_step_f64), bit-identical to sys() + scale_sum2/scale_sum5; and
+// rk4_step_into(x, out, t, dt): the same step with the same bits in one
+// temporally blocked kernel that reads x once and writes out once.
+// hip_runge_kutta4 (stepper.hpp) calls rk4_step_into.
 #pragma once
 
 #include <memory>
@@ -153,11 +152,8 @@ struct HipVariant {
     check(cf_variant_rk4_step(h_->get(), x.data(), t, dt, work_->data(), default_stream()), "rk4_step");
   }
 
-  // One temporally blocked RK4 step from x into out (same bits as rk4_step); the
-  // comoving variants rebuild Psi at every stage time and have no such kernel.
-  void rk4_step_into(const State &x, State &out, double t, double dt)
-    requires(KIND == CF_VAR_FIXED_CURVATURE || KIND == CF_VAR_SQRT_POTENTIAL)
-  {
+  // One temporally blocked RK4 step from x into out (same bits as rk4_step).
+  void rk4_step_into(const State &x, State &out, double t, double dt) {
     if (out.size() != x.size()) out.resize(x.size());
     check(cf_variant_rk4_step_tb(h_->get(), x.data(), out.data(), t, dt, 0, default_stream()), "rk4_step_into");
   }

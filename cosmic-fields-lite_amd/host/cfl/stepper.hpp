@@ -2,13 +2,12 @@
 //
 // hip_runge_kutta4 has the interface of an odeint explicit stepper (the shape
 // of the reference's own custom stepper, src/midpoint.hpp:25-101): do_step(
-// system, x, t, dt). If the system provides rk4_step_into(x, out, t, dt) (the
-// Klein-Gordon, lambda, fixed-curvature and sqrt-potential equations) the step
-// is one temporally blocked kernel from x into a spare state, and then x and the
-// spare trade buffers (x.swap), so x's storage alternates between two
-// allocations; else, if it provides rk4_step(x, t, dt) (the comoving
-// equations), four fused stage kernels in place; otherwise the classic sys() +
-// scale_sum sequence. All three give the same bits. Works with
+// system, x, t, dt). If the system provides rk4_step_into(x, out, t, dt) (every
+// HIP equation does) the step is one temporally blocked kernel from x into a
+// spare state, and then x and the spare trade buffers (x.swap), so x's storage
+// alternates between two allocations; else, if it provides rk4_step(x, t, dt),
+// four fused stage kernels in place; otherwise the classic sys() + scale_sum
+// sequence. All three give the same bits. Works with
 // cfl::odeint_lite::integrate_const and, when Boost.odeint is present, with
 // boost::numeric::odeint::integrate_const.
 //

@@ -252,9 +252,9 @@ cf_status cf_variant_rk4_step(cf_variant v, double *x_dev, double t, double dt, 
 cf_status cf_variant_rk4_run(cf_variant v, double *x_dev, double t0, int64_t k0, int64_t n_steps, double dt,
                              void *work_dev, cf_stream stream);
 /* The same steps temporally blocked (one kernel per step, as cf_rk4_run_tb_f64) for
- * CF_VAR_FIXED_CURVATURE and CF_VAR_SQRT_POTENTIAL; same bits as cf_variant_rk4_run.
- * work_dev: cf_rk4_tb_work_bytes(N, CF_PREC_F64). The comoving variants rebuild Psi at
- * every stage time and return CF_E_UNSUPPORTED. */
+ * every variant; same bits as cf_variant_rk4_run. The comoving variants build Psi,
+ * dPsi/dt at t, t+dt/2 and t+dt first (two new transforms per step, as the stage
+ * kernels). work_dev: cf_rk4_tb_work_bytes(N, CF_PREC_F64). */
 cf_status cf_variant_rk4_run_tb(cf_variant v, double *x_dev, double t0, int64_t k0, int64_t n_steps, double dt,
                                 int chunk, void *work_dev, cf_stream stream);
 cf_status cf_variant_rk4_step_tb(cf_variant v, const double *x_in_dev, double *x_out_dev, double t, double dt,

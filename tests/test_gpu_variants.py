@@ -195,11 +195,27 @@ def test_rk4_tb_equals_stage_kernels(cfl, O, name, kind, N, steps, chunk):
     v.close()
 
 
-def test_rk4_tb_comoving_unsupported(cfl, O):
-    v, _ = setup(cfl, O, O.COMOVING, 16)
-    x = dev(rand_state(16, 1, 0.3))
-    with pytest.raises(cfl.CfError):
-        v.rk4_run_tb(x, 1.0, 0, 1, 0.05)
-    with pytest.raises(cfl.CfError):
-        v.rk4_step_tb(x, torch.empty_like(x), 1.0, 0.05)
+@pytest.mark.parametrize("name,kind,N,M,steps", [("comoving", 3, 16, 0, 3), ("comoving", 3, 40, 0, 2),
+                                                  ("approx", 4, 16, 8, 3), ("approx", 4, 48, 16, 2),
+                                                  ("approx", 4, 24, 24, 1)])
+def test_rk4_tb_comoving_equals_stage_kernels(cfl, O, name, kind, N, M, steps):
+    """The temporally blocked comoving steps (Psi at the three stage times from the
+    cache, read per stage at the stage's plane) give the stage kernels' bits, and
+    stay within the variants' 1e-12 bar of the oracle."""
+    print(f"case rk4_tb {name} N={N} M={M} steps={steps}", flush=True)
+    v, ov = setup(cfl, O, kind, N, M=M)
+    x0 = rand_state(N, 13, 0.3)
+    xa, xb = dev(x0), dev(x0)
+    t0, dt = 1.0, 0.05
+    v.rk4_run(xa, t0, 1, steps, dt)
+    v2, _ = setup(cfl, O, kind, N, M=M)  # a fresh Psi cache
+    v2.rk4_run_tb(xb, t0, 1, steps, dt)
+    assert_bitwise(host(xb), host(xa), f"{name} tb vs stage kernels")
+    xc, xo = dev(x0), torch.empty_like(xa)
+    for s in range(steps):  # out-of-place single steps, sharing v's cache with rk4_run
+        v.rk4_step_tb(xc, xo, t0 + float(1 + s) * dt, dt)
+        xc, xo = xo, xc
+    assert_bitwise(host(xc), host(xa), f"{name} single tb steps vs stage kernels")
+    close(host(xb), ov.rk4_run(x0.copy(), t0, dt, steps, k0=1), f"{name} tb vs oracle")
     v.close()
+    v2.close()
