@@ -25,13 +25,16 @@
 // where f = [pi, -3H pi - m^2 phi - lambda phi^3 + inv Lap phi] and
 //   (J^T a)_phi = (-m^2 - 3 lambda phi^2) a_pi + inv (sum_nbr a_pi - 6 a_pi)
 //   (J^T a)_pi  = a_phi - 3H a_pi.
-// The stage states y_i are recomputed from x with the forward kernels (same bits).
+// The backward step runs as two column-marching kernels (k_adj_back_a, k_adj_back_b)
+// that recompute the stage states they need from x on the fly with the forward
+// stage arithmetic, in the forward kernels' operation order (same bits).
 //
 // Checkpointing: the forward pass keeps the state every `ckpt` steps; the
 // backward pass recomputes one segment of states at a time. Scalars (L, dL/dm)
 // come from fixed-grid block partials summed on the host in a fixed order, so
 // every result is bit-reproducible.
 #include <cmath>
+#include <utility>
 #include <vector>
 
 #include "cf_march.hpp"
@@ -55,71 +58,54 @@ __device__ __forceinline__ void block_partial(double v, double *part_slot, int t
   if (tid == 0) *part_slot = sm[0];
 }
 
-struct AdjCoef {
-  double c_h;   // -3H at the stage time
-  double c_m;   // m*m
-  double lam;   // lambda
-  double inv;   // inv_ah_sqr at the stage time
-  double two_m; // 2m
-  double ck;    // a = ck*lambda' + dk*ybar_prev
-  double dk;
+// Coefficients of one backward step from t: the forward right-hand side at t
+// (stage 1) and at t + dt/2 (stages 2 and 3), -3H and inv_ah_sqr at t + dt (stage 4).
+struct AdjStep {
+  RhsCoef r1, r2;
+  double ch4, inv4;
+  double dh, dt, dt3, dt6;  // dt/2, dt, dt/3, dt/6
+  double two_m;             // 2m
 };
 
-// Stage-state recompute: y = x + cy * f(in) (the forward stage kernels' arithmetic,
-// so y1..y3 equal the forward step's stage states bit for bit). One thread per (b,c)
-// column marching along a, as the forward stage kernels.
-template <bool LAM>
-__global__ __launch_bounds__(ADJ_THREADS) void k_adj_y(Tiling tl, RhsCoef rk, const double *__restrict__ in,
-                                                       const double *__restrict__ x, double *__restrict__ y,
-                                                       double cy) {
-  int ct, bt, ac;
-  tile_of(tl, ct, bt, ac);
-  const int64_t N = tl.N, NN = N * N, N3 = NN * N;
-  const int64_t c = (int64_t)ct * TC + threadIdx.x;
-  const int64_t b = (int64_t)bt * TB + threadIdx.y;
-  const int64_t a0 = (int64_t)ac * tl.A;
-  if (c >= N || b >= N || a0 >= N) return;
-  const int64_t a1 = (a0 + tl.A < N) ? a0 + tl.A : N;
-  const int64_t cp = (c + 1 == N) ? 0 : c + 1, cm = (c == 0) ? N - 1 : c - 1;
-  const int64_t bp = (b + 1 == N) ? 0 : b + 1, bm = (b == 0) ? N - 1 : b - 1;
-  const int64_t col = b * N + c;
-  double prev = in[((a0 == 0) ? N - 1 : a0 - 1) * NN + col];
-  double cur = in[a0 * NN + col];
-  for (int64_t a = a0; a < a1; ++a) {
-    const int64_t ap = (a + 1 == N) ? 0 : a + 1;
-    const int64_t row = a * NN, i = row + col;
-    const double next = in[ap * NN + col];
-    const double piv = in[N3 + i];
-    const double kpi = rhs_pi(cur, next, prev, in[row + bp * N + c], in[row + bm * N + c], in[row + b * N + cp],
-                              in[row + b * N + cm], piv, rk, LAM);
-    y[i] = x[i] + cy * piv;
-    y[N3 + i] = x[N3 + i] + cy * kpi;
-    prev = cur;
-    cur = next;
-  }
+// The transposed Jacobian applied to a at a site, phi component:
+// (-m^2 - 3 lambda phi_y^2) a_pi + inv (sum of the six neighbours' a_pi - 6 a_pi).
+__device__ __forceinline__ double jt_phi(double c_m, double lam, double phi_y, double inv, double a0, double nb) {
+  return (-c_m - 3.0 * lam * phi_y * phi_y) * a0 + inv * (nb - 6.0 * a0);
 }
 
-// One backward stage: a = ck*lp + dk*yb_prev (HAS_YB) or ck*lp; out = J(y)^T a.
-// MODE 0 and 1: ybar = out (each stage's output kept in its own buffer); MODE 2 (the
-// last): acc = (((lp + yb4) + yb3) + yb_prev) + out, the lambda of the step in the
-// order of a running sum, without reading and rewriting an accumulator per stage.
-// part[blockIdx] = sum over the block's sites of (-2 m phi_y) a_pi, in a fixed order.
-// A fixed grid of ADJ_BLOCKS blocks walks the tiles (one thread per (b,c) column
-// marching along a, a_pi at a-1, a, a+1 kept in registers).
-template <int MODE, bool HAS_YB>
-__global__ __launch_bounds__(ADJ_THREADS) void k_adj_back(Tiling tl, const double *__restrict__ lp,
-                                                          const double *__restrict__ ybp,
-                                                          const double *__restrict__ y, double *__restrict__ ybar,
-                                                          double *__restrict__ acc, const double *__restrict__ yb4,
-                                                          const double *__restrict__ yb3, AdjCoef k,
-                                                          double *__restrict__ part) {
+// Backward step, first half: stages 4, 3 and the local part of stage 2.
+//
+// Only phi of each stage state enters the transposed Jacobian, and only at the
+// site, and phi of a stage state is local in the previous one:
+//   y1.phi = x.phi + h/2 x.pi,  y2.phi = x.phi + h/2 y1.pi,  y3.phi = x.phi + h y2.pi
+// with y1.pi = x.pi + h/2 f_pi(x) and y2.pi = x.pi + h/2 f_pi(y1): every y_s.phi at
+// a site follows from x within distance 1 (the forward stage arithmetic, so the
+// same bits as the stored stage states). Likewise, with ybar_s = J(y_s)^T a_{s+1},
+// ybar.pi = a.phi - 3H a.pi is local and only ybar.phi has a stencil, so
+//   a4 = h/6 lambda'                         (local)
+//   ybar3 (at y3)                            phi: stencil of a4.pi
+//   a3 = h/3 lambda' + h ybar3               pi: local
+//   ybar2 (at y2)                            phi: stencil of a3.pi
+//   a2 = h/3 lambda' + h/2 ybar2             at the site
+//   ybar1.pi = a2.phi - 3H a2.pi,  a1.pi = h/6 lambda'.pi + h/2 ybar1.pi
+// all within distance 1 of lambda' and x. Writes s = (lambda' + ybar3) + ybar2 (the
+// pi half also + ybar1.pi, the order of the running sum) and w = (a2.pi, a1.pi),
+// the two fields the second half takes stencils of. part: stage 4, 3, 2 partials
+// of dL/dm, ADJ_BLOCKS apart. One thread per (b,c) column marching along a; a fixed
+// grid of ADJ_BLOCKS blocks walks the tiles so the partials have a fixed order.
+template <bool LAM>
+__global__ __launch_bounds__(ADJ_THREADS) void k_adj_back_a(Tiling tl, AdjStep k, const double *__restrict__ x,
+                                                            const double *__restrict__ lp, double *__restrict__ s,
+                                                            double *__restrict__ w, double *__restrict__ part) {
   const int64_t N = tl.N, NN = N * N, N3 = NN * N;
-  auto api = [&](int64_t j) {
-    const double v = k.ck * lp[N3 + j];
-    if constexpr (HAS_YB) return v + k.dk * ybp[N3 + j];
-    else return v;
+  const double c_m = k.r1.c_m, lam = k.r1.lam;
+  auto y1phi = [&](int64_t j) { return x[j] + k.dh * x[N3 + j]; };
+  auto a4pi = [&](int64_t j) { return k.dt6 * lp[N3 + j]; };
+  auto a3pi = [&](int64_t j) {
+    const double yb4pi = k.dt6 * lp[j] + k.ch4 * a4pi(j);
+    return k.dt3 * lp[N3 + j] + k.dt * yb4pi;
   };
-  double dm = 0.0;
+  double dm4 = 0.0, dm3 = 0.0, dm2 = 0.0;
   for (int t = blockIdx.x; t < tl.total; t += ADJ_BLOCKS) {
     const int ct = t % tl.n_ct, r = t / tl.n_ct, bt = r % tl.n_bt, ac = r / tl.n_bt;
     const int64_t c = (int64_t)ct * TC + threadIdx.x;
@@ -130,33 +116,98 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_back(Tiling tl, const doubl
     const int64_t cp = (c + 1 == N) ? 0 : c + 1, cm = (c == 0) ? N - 1 : c - 1;
     const int64_t bp = (b + 1 == N) ? 0 : b + 1, bm = (b == 0) ? N - 1 : b - 1;
     const int64_t col = b * N + c;
-    double prev = api(((a0 == 0) ? N - 1 : a0 - 1) * NN + col);
-    double cur = api(a0 * NN + col);
+    const int64_t jp = ((a0 == 0) ? N - 1 : a0 - 1) * NN + col, jc = a0 * NN + col;
+    double xp = x[jp], xc = x[jc];
+    double yp = y1phi(jp), yc = y1phi(jc);
+    double fp = a4pi(jp), fc = a4pi(jc);
+    double gp = a3pi(jp), gc = a3pi(jc);
     for (int64_t a = a0; a < a1; ++a) {
       const int64_t ap = (a + 1 == N) ? 0 : a + 1;
-      const int64_t row = a * NN, i = row + col;
-      const double next = api(ap * NN + col);
-      double aphi = k.ck * lp[i];
-      if constexpr (HAS_YB) aphi = aphi + k.dk * ybp[i];
-      const double a0v = cur;
-      const double nb = next + prev + api(row + bp * N + c) + api(row + bm * N + c) + api(row + b * N + cp) +
-                        api(row + b * N + cm);
-      const double phi = y[i];
-      const double out_phi = (-k.c_m - 3.0 * k.lam * phi * phi) * a0v + k.inv * (nb - 6.0 * a0v);
-      const double out_pi = aphi + k.c_h * a0v;
-      if constexpr (MODE == 2) {
-        acc[i] = (((lp[i] + yb4[i]) + yb3[i]) + ybp[i]) + out_phi;
-        acc[N3 + i] = (((lp[N3 + i] + yb4[N3 + i]) + yb3[N3 + i]) + ybp[N3 + i]) + out_pi;
-      } else {
-        ybar[i] = out_phi;
-        ybar[N3 + i] = out_pi;
-      }
-      dm += (-k.two_m * phi) * a0v;
-      prev = cur;
-      cur = next;
+      const int64_t row = a * NN, i = row + col, jn = ap * NN + col;
+      const int64_t j0 = row + bp * N + c, j1 = row + bm * N + c, j2 = row + b * N + cp, j3 = row + b * N + cm;
+      const double xn = x[jn], yn = y1phi(jn), fn = a4pi(jn), gn = a3pi(jn);
+      const double xpi = x[N3 + i], lphi = lp[i], lpi = lp[N3 + i];
+      // stage states at the site
+      const double y1pi = xpi + k.dh * rhs_pi(xc, xn, xp, x[j0], x[j1], x[j2], x[j3], xpi, k.r1, LAM);
+      const double y2pi =
+          xpi + k.dh * rhs_pi(yc, yn, yp, y1phi(j0), y1phi(j1), y1phi(j2), y1phi(j3), y1pi, k.r2, LAM);
+      const double y2phi = xc + k.dh * y1pi;
+      const double y3phi = xc + k.dt * y2pi;
+      // stage 4 at y3 (t + dt)
+      const double nb4 = fn + fp + a4pi(j0) + a4pi(j1) + a4pi(j2) + a4pi(j3);
+      const double yb4phi = jt_phi(c_m, lam, y3phi, k.inv4, fc, nb4);
+      const double yb4pi = k.dt6 * lphi + k.ch4 * fc;
+      dm4 += (-k.two_m * y3phi) * fc;
+      // stage 3 at y2 (t + dt/2)
+      const double nb3 = gn + gp + a3pi(j0) + a3pi(j1) + a3pi(j2) + a3pi(j3);
+      const double yb3phi = jt_phi(c_m, lam, y2phi, k.r2.inv, gc, nb3);
+      const double yb3pi = (k.dt3 * lphi + k.dt * yb4phi) + k.r2.c_h * gc;
+      dm3 += (-k.two_m * y2phi) * gc;
+      // stage 2 at y1 (t + dt/2): the local part
+      const double a2pi = k.dt3 * lpi + k.dh * yb3pi;
+      const double a2phi = k.dt3 * lphi + k.dh * yb3phi;
+      const double yb2pi = a2phi + k.r2.c_h * a2pi;
+      dm2 += (-k.two_m * yc) * a2pi;
+      s[i] = (lphi + yb4phi) + yb3phi;
+      s[N3 + i] = ((lpi + yb4pi) + yb3pi) + yb2pi;
+      w[i] = a2pi;
+      w[N3 + i] = k.dt6 * lpi + k.dh * yb2pi;
+      xp = xc; xc = xn;
+      yp = yc; yc = yn;
+      fp = fc; fc = fn;
+      gp = gc; gc = gn;
     }
   }
-  block_partial(dm, part + blockIdx.x, (int)(threadIdx.y * TC + threadIdx.x));
+  const int tid = (int)(threadIdx.y * TC + threadIdx.x);
+  block_partial(dm4, part + blockIdx.x, tid);
+  block_partial(dm3, part + ADJ_BLOCKS + blockIdx.x, tid);
+  block_partial(dm2, part + 2 * ADJ_BLOCKS + blockIdx.x, tid);
+}
+
+// Backward step, second half: ybar1.phi (at y1, stencil of a2.pi) and J(x)^T a1
+// (stencil of a1.pi), and lambda = ((s + ybar1) + J(x)^T a1); part: the stage 1
+// partials of dL/dm. lo must not overlap x, lp, s or w.
+__global__ __launch_bounds__(ADJ_THREADS) void k_adj_back_b(Tiling tl, AdjStep k, const double *__restrict__ x,
+                                                            const double *__restrict__ lp,
+                                                            const double *__restrict__ s,
+                                                            const double *__restrict__ w, double *__restrict__ lo,
+                                                            double *__restrict__ part) {
+  const int64_t N = tl.N, NN = N * N, N3 = NN * N;
+  const double c_m = k.r1.c_m, lam = k.r1.lam;
+  double dm1 = 0.0;
+  for (int t = blockIdx.x; t < tl.total; t += ADJ_BLOCKS) {
+    const int ct = t % tl.n_ct, r = t / tl.n_ct, bt = r % tl.n_bt, ac = r / tl.n_bt;
+    const int64_t c = (int64_t)ct * TC + threadIdx.x;
+    const int64_t b = (int64_t)bt * TB + threadIdx.y;
+    const int64_t a0 = (int64_t)ac * tl.A;
+    if (c >= N || b >= N || a0 >= N) continue;
+    const int64_t a1 = (a0 + tl.A < N) ? a0 + tl.A : N;
+    const int64_t cp = (c + 1 == N) ? 0 : c + 1, cm = (c == 0) ? N - 1 : c - 1;
+    const int64_t bp = (b + 1 == N) ? 0 : b + 1, bm = (b == 0) ? N - 1 : b - 1;
+    const int64_t col = b * N + c;
+    const int64_t jp = ((a0 == 0) ? N - 1 : a0 - 1) * NN + col, jc = a0 * NN + col;
+    double up = w[jp], uc = w[jc];            // a2.pi
+    double vp = w[N3 + jp], vc = w[N3 + jc];  // a1.pi
+    for (int64_t a = a0; a < a1; ++a) {
+      const int64_t ap = (a + 1 == N) ? 0 : a + 1;
+      const int64_t row = a * NN, i = row + col, jn = ap * NN + col;
+      const int64_t j0 = row + bp * N + c, j1 = row + bm * N + c, j2 = row + b * N + cp, j3 = row + b * N + cm;
+      const double un = w[jn], vn = w[N3 + jn];
+      const double xphi = x[i];
+      const double y1 = xphi + k.dh * x[N3 + i];
+      const double nb2 = un + up + w[j0] + w[j1] + w[j2] + w[j3];
+      const double yb2phi = jt_phi(c_m, lam, y1, k.r2.inv, uc, nb2);
+      const double nb1 = vn + vp + w[N3 + j0] + w[N3 + j1] + w[N3 + j2] + w[N3 + j3];
+      const double outphi = jt_phi(c_m, lam, xphi, k.r1.inv, vc, nb1);
+      const double outpi = (k.dt6 * lp[i] + k.dh * yb2phi) + k.r1.c_h * vc;
+      lo[i] = (s[i] + yb2phi) + outphi;
+      lo[N3 + i] = s[N3 + i] + outpi;
+      dm1 += (-k.two_m * xphi) * vc;
+      up = uc; uc = un;
+      vp = vc; vc = vn;
+    }
+  }
+  block_partial(dm1, part + blockIdx.x, (int)(threadIdx.y * TC + threadIdx.x));
 }
 
 // Half-spectrum fk *= 2 w[s].
@@ -215,8 +266,8 @@ size_t cf_adjoint_work_bytes(int64_t N, int64_t n_steps, int64_t ckpt) {
   if (!valid_N(N) || n_steps < 0 || ckpt <= 0) return 0;
   const size_t v = 2 * (size_t)N * N * N * sizeof(double);
   const size_t nck = (size_t)((n_steps + ckpt - 1) / ckpt);
-  // checkpoints + one segment of states + y1,y2,y3 + k + lambda, lambda', ybar x2, acc + rk4 work (3)
-  return nck * v + (size_t)ckpt * v + 3 * v + v + 5 * v + 3 * v +
+  // checkpoints + one segment of states + step work (2) + lambda (2) + rk4 work (3)
+  return nck * v + (size_t)ckpt * v + 2 * v + 2 * v + 3 * v +
          (size_t)(4 * n_steps + 1) * ADJ_BLOCKS * sizeof(double);
 }
 
@@ -272,63 +323,57 @@ cf_status cf_loss_spectrum_adjoint(cf_spectrum_plan *plan, const cf_kg_params *p
   return CF_OK;
 }
 
+// One backward step from state x at time t: lo = dL/dx_n from li = dL/dx_{n+1};
+// part: 4*ADJ_BLOCKS partials of dL/dm (stages 4, 3, 2, 1). sw, ww: 2N^3 doubles
+// each. lo overlaps none of x, li, sw, ww.
+static cf_status adj_step(const cf_kg_params *p, const double *x, double t, double dt, const double *li, double *lo,
+                          double *part, double *sw, double *ww, hipStream_t st) {
+  const int64_t N = p->N;
+  const double dh = dt / 2.0, th = t + dh;
+  auto rhs = [&](double tt) {
+    RhsCoef r;
+    r.c_h = (-3.0) * cf_cosmo_H(&p->cosmo, tt);
+    r.c_m = p->m * p->m;
+    r.lam = p->lambda;
+    r.inv = cf_inv_ah_sqr(N, p->L, cf_cosmo_a(&p->cosmo, tt));
+    return r;
+  };
+  AdjStep k;
+  k.r1 = rhs(t);
+  k.r2 = rhs(th);
+  const RhsCoef r4 = rhs(t + dt);
+  k.ch4 = r4.c_h;
+  k.inv4 = r4.inv;
+  k.dh = dh;
+  k.dt = dt;
+  k.dt3 = dt / 3.0;
+  k.dt6 = dt / 6.0;
+  k.two_m = 2.0 * p->m;
+  const Tiling tl = make_tiling(N, default_chunk(N));
+  const dim3 g(ADJ_BLOCKS), blk(TC, TB);
+  static_assert(TC * TB == ADJ_THREADS, "block shape");
+  if (p->lambda != 0.0)
+    hipLaunchKernelGGL(k_adj_back_a<true>, g, blk, 0, st, tl, k, x, li, sw, ww, part);
+  else
+    hipLaunchKernelGGL(k_adj_back_a<false>, g, blk, 0, st, tl, k, x, li, sw, ww, part);
+  hipLaunchKernelGGL(k_adj_back_b, g, blk, 0, st, tl, k, x, li, (const double *)sw, (const double *)ww, lo,
+                     part + 3 * ADJ_BLOCKS);
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
 // One RK4 step adjoint at state x_n (time t): lam_dev holds dL/dx_{n+1} on entry and
 // dL/dx_n on return; the step's dL/dm contributions go to part_dev (4*ADJ_BLOCKS
-// doubles, summed by the caller). work_dev: 9 vectors of 2N^3 doubles.
+// doubles, summed by the caller). work_dev: 3 vectors of 2N^3 doubles.
 cf_status cf_rk4_step_adjoint_f64(const cf_kg_params *p, const double *x, double t, double dt, double *lam,
                                   double *part_dev, void *work_dev, cf_stream stream) {
   CF_REQUIRE(p && x && lam && part_dev && work_dev, "cf_rk4_step_adjoint_f64: null argument");
   CF_REQUIRE(valid_N(p->N), "cf_rk4_step_adjoint_f64: unsupported N");
   const int64_t N = p->N, n = 2 * N * N * N;
   hipStream_t st = S(stream);
-  double *y1 = (double *)work_dev, *y2 = y1 + n, *y3 = y2 + n;
-  double *yb4 = y3 + n, *yb3 = yb4 + n, *yb2 = yb3 + n, *acc = yb2 + n;
-  const double dh = dt / 2.0, th = t + dh, dt6 = dt / 6.0, dt3 = dt / 3.0;
-  const Tiling tl = make_tiling(N, default_chunk(N));
-  const dim3 blk(TC, TB);
-  static_assert(TC * TB == ADJ_THREADS, "block shape");
-  // recompute the stage states with the forward stage arithmetic (same bits)
-  auto ystage = [&](const double *in, double tt, double cy, double *y) -> cf_status {
-    const double a_t = cf_cosmo_a(&p->cosmo, tt);
-    RhsCoef rk;
-    rk.c_h = (-3.0) * cf_cosmo_H(&p->cosmo, tt);
-    rk.c_m = p->m * p->m;
-    rk.lam = p->lambda;
-    rk.inv = cf_inv_ah_sqr(N, p->L, a_t);
-    if (rk.lam != 0.0)
-      hipLaunchKernelGGL(k_adj_y<true>, dim3(tl.total), blk, 0, st, tl, rk, in, x, y, cy);
-    else
-      hipLaunchKernelGGL(k_adj_y<false>, dim3(tl.total), blk, 0, st, tl, rk, in, x, y, cy);
-    CF_LAUNCHED();
-    return CF_OK;
-  };
-  CF_TRY(ystage(x, t, dh, y1));
-  CF_TRY(ystage(y1, th, dh, y2));
-  CF_TRY(ystage(y2, th, dt, y3));
-  auto coef = [&](double tt, double ck, double dk) {
-    AdjCoef k;
-    const double a_t = cf_cosmo_a(&p->cosmo, tt);
-    k.c_h = (-3.0) * cf_cosmo_H(&p->cosmo, tt);
-    k.c_m = p->m * p->m;
-    k.lam = p->lambda;
-    k.inv = cf_inv_ah_sqr(N, p->L, a_t);
-    k.two_m = 2.0 * p->m;
-    k.ck = ck;
-    k.dk = dk;
-    return k;
-  };
-  const dim3 g(ADJ_BLOCKS);
-  const double *no = nullptr;
-  hipLaunchKernelGGL((k_adj_back<0, false>), g, blk, 0, st, tl, lam, no, y3, yb4, (double *)nullptr, no, no,
-                     coef(t + dt, dt6, 0.0), part_dev);
-  hipLaunchKernelGGL((k_adj_back<1, true>), g, blk, 0, st, tl, lam, yb4, y2, yb3, (double *)nullptr, no, no,
-                     coef(th, dt3, dt), part_dev + ADJ_BLOCKS);
-  hipLaunchKernelGGL((k_adj_back<1, true>), g, blk, 0, st, tl, lam, yb3, y1, yb2, (double *)nullptr, no, no,
-                     coef(th, dt3, dh), part_dev + 2 * ADJ_BLOCKS);
-  hipLaunchKernelGGL((k_adj_back<2, true>), g, blk, 0, st, tl, lam, yb2, x, (double *)nullptr, acc, yb4, yb3,
-                     coef(t, dt6, dh), part_dev + 3 * ADJ_BLOCKS);
-  CF_LAUNCHED();
-  CF_HIP(hipMemcpyAsync(lam, acc, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  double *sw = (double *)work_dev, *ww = sw + n, *lo = ww + n;
+  CF_TRY(adj_step(p, x, t, dt, lam, lo, part_dev, sw, ww, st));
+  CF_HIP(hipMemcpyAsync(lam, lo, n * sizeof(double), hipMemcpyDeviceToDevice, st));
   return CF_OK;
 }
 
@@ -348,11 +393,12 @@ cf_status cf_adjoint_spectrum_loss(cf_spectrum_plan *plan, const cf_kg_params *p
   const int64_t nck = (n_steps + ckpt - 1) / ckpt;
   double *ck = (double *)work_dev;                // nck checkpoints
   double *seg = ck + (size_t)nck * n;              // ckpt states of one segment
-  double *sw = seg + (size_t)ckpt * n;             // step-adjoint work: 7 vectors
-  double *lamb = sw + 7 * n;                       // lambda
-  double *fw = lamb + n;                           // forward work: 3 vectors
-  double *parts = fw + 3 * n;                      // (4 n_steps + 1) * ADJ_BLOCKS
-  (void)parts;
+  double *sw = seg + (size_t)ckpt * n;             // step-adjoint work: 2 vectors
+  double *ww = sw + n;
+  double *lamb = ww + n;                           // lambda, two buffers taken in turn
+  double *lamb2 = lamb + n;
+  double *fw = lamb2 + n;                          // forward work: 3 vectors
+  double *part = fw + 3 * n;                       // (4 n_steps + 1) * ADJ_BLOCKS
   // forward with checkpoints
   double *x = seg;  // running state
   CF_HIP(hipMemcpyAsync(x, x0, vb, hipMemcpyDeviceToDevice, st));
@@ -364,7 +410,6 @@ cf_status cf_adjoint_spectrum_loss(cf_spectrum_plan *plan, const cf_kg_params *p
   const double t_end = t0 + static_cast<double>(n_steps) * dt;
   double dm_loss = 0.0;
   CF_TRY(cf_loss_spectrum_adjoint(plan, p, x, t_end, weights_dev, lamb, fw, loss, &dm_loss, stream));
-  double *part = (double *)(fw + 3 * n);
   // backward, one segment at a time
   for (int64_t c = nck - 1; c >= 0; --c) {
     const int64_t k0 = c * ckpt, k1 = (k0 + ckpt < n_steps) ? k0 + ckpt : n_steps;
@@ -376,8 +421,9 @@ cf_status cf_adjoint_spectrum_loss(cf_spectrum_plan *plan, const cf_kg_params *p
     }
     for (int64_t k = k1 - 1; k >= k0; --k) {
       const double t = t0 + static_cast<double>(k) * dt;
-      CF_TRY(cf_rk4_step_adjoint_f64(p, seg + (size_t)(k - k0) * n, t, dt, lamb, part + (size_t)k * 4 * ADJ_BLOCKS,
-                                     sw, stream));
+      CF_TRY(adj_step(p, seg + (size_t)(k - k0) * n, t, dt, lamb, lamb2, part + (size_t)k * 4 * ADJ_BLOCKS, sw, ww,
+                      st));
+      std::swap(lamb, lamb2);
     }
   }
   std::vector<double> hp((size_t)n_steps * 4 * ADJ_BLOCKS);
