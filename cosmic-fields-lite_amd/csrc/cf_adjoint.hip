@@ -67,6 +67,14 @@ struct AdjStep {
   double two_m;             // 2m
 };
 
+// Tile of the v-th slot of the fixed grid's walk. Blocks go to the 8 XCDs round
+// robin (block mod 8), and a block's slots keep its residue, so when the tile
+// count allows, each XCD is given one contiguous range of tiles: the halo rows
+// and planes that neighbouring tiles share meet in one L2.
+__device__ __forceinline__ int adj_tile(int v, int total) {
+  return (total & 7) == 0 ? (v & 7) * (total >> 3) + (v >> 3) : v;
+}
+
 // The transposed Jacobian applied to a at a site, phi component:
 // (-m^2 - 3 lambda phi_y^2) a_pi + inv (sum of the six neighbours' a_pi - 6 a_pi).
 __device__ __forceinline__ double jt_phi(double c_m, double lam, double phi_y, double inv, double a0, double nb) {
@@ -99,14 +107,16 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_back_a(Tiling tl, AdjStep k
                                                             double *__restrict__ w, double *__restrict__ part) {
   const int64_t N = tl.N, NN = N * N, N3 = NN * N;
   const double c_m = k.r1.c_m, lam = k.r1.lam;
-  auto y1phi = [&](int64_t j) { return x[j] + k.dh * x[N3 + j]; };
-  auto a4pi = [&](int64_t j) { return k.dt6 * lp[N3 + j]; };
-  auto a3pi = [&](int64_t j) {
-    const double yb4pi = k.dt6 * lp[j] + k.ch4 * a4pi(j);
-    return k.dt3 * lp[N3 + j] + k.dt * yb4pi;
+  // per-point fields from loaded (x.phi, x.pi) and (lambda'.phi, lambda'.pi)
+  auto y1phi = [&](double f, double p) { return f + k.dh * p; };
+  auto a4pi = [&](double lpi) { return k.dt6 * lpi; };
+  auto a3pi = [&](double lphi, double lpi) {
+    const double yb4pi = k.dt6 * lphi + k.ch4 * a4pi(lpi);
+    return k.dt3 * lpi + k.dt * yb4pi;
   };
   double dm4 = 0.0, dm3 = 0.0, dm2 = 0.0;
-  for (int t = blockIdx.x; t < tl.total; t += ADJ_BLOCKS) {
+  for (int v = blockIdx.x; v < tl.total; v += ADJ_BLOCKS) {
+    const int t = adj_tile(v, tl.total);
     const int ct = t % tl.n_ct, r = t / tl.n_ct, bt = r % tl.n_bt, ac = r / tl.n_bt;
     const int64_t c = (int64_t)ct * TC + threadIdx.x;
     const int64_t b = (int64_t)bt * TB + threadIdx.y;
@@ -116,46 +126,75 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_back_a(Tiling tl, AdjStep k
     const int64_t cp = (c + 1 == N) ? 0 : c + 1, cm = (c == 0) ? N - 1 : c - 1;
     const int64_t bp = (b + 1 == N) ? 0 : b + 1, bm = (b == 0) ? N - 1 : b - 1;
     const int64_t col = b * N + c;
+    const int64_t nbr[4] = {bp * N + c, bm * N + c, b * N + cp, b * N + cm};  // in-plane neighbours
     const int64_t jp = ((a0 == 0) ? N - 1 : a0 - 1) * NN + col, jc = a0 * NN + col;
-    double xp = x[jp], xc = x[jc];
-    double yp = y1phi(jp), yc = y1phi(jc);
-    double fp = a4pi(jp), fc = a4pi(jc);
-    double gp = a3pi(jp), gc = a3pi(jc);
-    for (int64_t a = a0; a < a1; ++a) {
+    double xp = x[jp], xc = x[jc], xpic = x[N3 + jc], lphic = lp[jc], lpic = lp[N3 + jc];
+    double yp = y1phi(xp, x[N3 + jp]), yc = y1phi(xc, xpic);
+    double fp = a4pi(lp[N3 + jp]), fc = a4pi(lpic);
+    double gp = a3pi(lp[jp], lp[N3 + jp]), gc = a3pi(lphic, lpic);
+    // the loads of plane a: (x.phi, x.pi, lambda'.phi, lambda'.pi) at a+1 on the
+    // column (e[0..3]) and at plane a's four in-plane neighbours (e[4..19]); those of
+    // the next plane are issued before plane a's arithmetic, so that they overlap it
+    auto load_plane = [&](int64_t a, double *e) {
       const int64_t ap = (a + 1 == N) ? 0 : a + 1;
-      const int64_t row = a * NN, i = row + col, jn = ap * NN + col;
-      const int64_t j0 = row + bp * N + c, j1 = row + bm * N + c, j2 = row + b * N + cp, j3 = row + b * N + cm;
-      const double xn = x[jn], yn = y1phi(jn), fn = a4pi(jn), gn = a3pi(jn);
-      const double xpi = x[N3 + i], lphi = lp[i], lpi = lp[N3 + i];
-      // stage states at the site
-      const double y1pi = xpi + k.dh * rhs_pi(xc, xn, xp, x[j0], x[j1], x[j2], x[j3], xpi, k.r1, LAM);
-      const double y2pi =
-          xpi + k.dh * rhs_pi(yc, yn, yp, y1phi(j0), y1phi(j1), y1phi(j2), y1phi(j3), y1pi, k.r2, LAM);
-      const double y2phi = xc + k.dh * y1pi;
-      const double y3phi = xc + k.dt * y2pi;
-      // stage 4 at y3 (t + dt)
-      const double nb4 = fn + fp + a4pi(j0) + a4pi(j1) + a4pi(j2) + a4pi(j3);
-      const double yb4phi = jt_phi(c_m, lam, y3phi, k.inv4, fc, nb4);
-      const double yb4pi = k.dt6 * lphi + k.ch4 * fc;
-      dm4 += (-k.two_m * y3phi) * fc;
-      // stage 3 at y2 (t + dt/2)
-      const double nb3 = gn + gp + a3pi(j0) + a3pi(j1) + a3pi(j2) + a3pi(j3);
-      const double yb3phi = jt_phi(c_m, lam, y2phi, k.r2.inv, gc, nb3);
-      const double yb3pi = (k.dt3 * lphi + k.dt * yb4phi) + k.r2.c_h * gc;
-      dm3 += (-k.two_m * y2phi) * gc;
-      // stage 2 at y1 (t + dt/2): the local part
-      const double a2pi = k.dt3 * lpi + k.dh * yb3pi;
-      const double a2phi = k.dt3 * lphi + k.dh * yb3phi;
-      const double yb2pi = a2phi + k.r2.c_h * a2pi;
-      dm2 += (-k.two_m * yc) * a2pi;
-      s[i] = (lphi + yb4phi) + yb3phi;
-      s[N3 + i] = ((lpi + yb4pi) + yb3pi) + yb2pi;
-      w[i] = a2pi;
-      w[N3 + i] = k.dt6 * lpi + k.dh * yb2pi;
-      xp = xc; xc = xn;
-      yp = yc; yc = yn;
-      fp = fc; fc = fn;
-      gp = gc; gc = gn;
+      const int64_t row = a * NN, jn = ap * NN + col;
+      e[0] = x[jn];
+      e[1] = x[N3 + jn];
+      e[2] = lp[jn];
+      e[3] = lp[N3 + jn];
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        e[4 + d] = x[row + nbr[d]];
+        e[8 + d] = x[N3 + row + nbr[d]];
+        e[12 + d] = lp[row + nbr[d]];
+        e[16 + d] = lp[N3 + row + nbr[d]];
+      }
+    };
+    // plane a from e (loaded earlier) while plane a+1 loads into en; unrolled by two
+    // so that the two buffers trade places without copies
+    auto plane = [&](int64_t a, const double *e, double *en) {
+      const int64_t row = a * NN, i = row + col;
+      load_plane((a + 1 < a1) ? a + 1 : a, en);
+      __builtin_amdgcn_sched_barrier(0);
+        const double xn = e[0], xpin = e[1], lphin = e[2], lpin = e[3];
+        const double *qf = e + 4, *qp = e + 8, *ql = e + 12, *qm = e + 16;
+        const double yn = y1phi(xn, xpin), fn = a4pi(lpin), gn = a3pi(lphin, lpin);
+        // stage states at the site
+        const double y1pi = xpic + k.dh * rhs_pi(xc, xn, xp, qf[0], qf[1], qf[2], qf[3], xpic, k.r1, LAM);
+        const double y2pi = xpic + k.dh * rhs_pi(yc, yn, yp, y1phi(qf[0], qp[0]), y1phi(qf[1], qp[1]),
+                                                            y1phi(qf[2], qp[2]), y1phi(qf[3], qp[3]), y1pi, k.r2, LAM);
+        const double y2phi = xc + k.dh * y1pi;
+        const double y3phi = xc + k.dt * y2pi;
+        // stage 4 at y3 (t + dt)
+        const double nb4 = fn + fp + a4pi(qm[0]) + a4pi(qm[1]) + a4pi(qm[2]) + a4pi(qm[3]);
+        const double yb4phi = jt_phi(c_m, lam, y3phi, k.inv4, fc, nb4);
+        const double yb4pi = k.dt6 * lphic + k.ch4 * fc;
+        dm4 += (-k.two_m * y3phi) * fc;
+        // stage 3 at y2 (t + dt/2)
+        const double nb3 =
+            gn + gp + a3pi(ql[0], qm[0]) + a3pi(ql[1], qm[1]) + a3pi(ql[2], qm[2]) + a3pi(ql[3], qm[3]);
+        const double yb3phi = jt_phi(c_m, lam, y2phi, k.r2.inv, gc, nb3);
+        const double yb3pi = (k.dt3 * lphic + k.dt * yb4phi) + k.r2.c_h * gc;
+        dm3 += (-k.two_m * y2phi) * gc;
+        // stage 2 at y1 (t + dt/2): the local part
+        const double a2pi = k.dt3 * lpic + k.dh * yb3pi;
+        const double a2phi = k.dt3 * lphic + k.dh * yb3phi;
+        const double yb2pi = a2phi + k.r2.c_h * a2pi;
+        dm2 += (-k.two_m * yc) * a2pi;
+        s[i] = (lphic + yb4phi) + yb3phi;
+        s[N3 + i] = ((lpic + yb4pi) + yb3pi) + yb2pi;
+        w[i] = a2pi;
+        w[N3 + i] = k.dt6 * lpic + k.dh * yb2pi;
+        xp = xc; xc = xn; xpic = xpin; lphic = lphin; lpic = lpin;
+        yp = yc; yc = yn;
+        fp = fc; fc = fn;
+        gp = gc; gc = gn;
+    };
+    double e0[20], e1[20];
+    load_plane(a0, e0);
+    for (int64_t a = a0; a < a1; a += 2) {
+      plane(a, e0, e1);
+      if (a + 1 < a1) plane(a + 1, e1, e0);
     }
   }
   const int tid = (int)(threadIdx.y * TC + threadIdx.x);
@@ -175,7 +214,8 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_back_b(Tiling tl, AdjStep k
   const int64_t N = tl.N, NN = N * N, N3 = NN * N;
   const double c_m = k.r1.c_m, lam = k.r1.lam;
   double dm1 = 0.0;
-  for (int t = blockIdx.x; t < tl.total; t += ADJ_BLOCKS) {
+  for (int v = blockIdx.x; v < tl.total; v += ADJ_BLOCKS) {
+    const int t = adj_tile(v, tl.total);
     const int ct = t % tl.n_ct, r = t / tl.n_ct, bt = r % tl.n_bt, ac = r / tl.n_bt;
     const int64_t c = (int64_t)ct * TC + threadIdx.x;
     const int64_t b = (int64_t)bt * TB + threadIdx.y;
@@ -186,22 +226,30 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_back_b(Tiling tl, AdjStep k
     const int64_t bp = (b + 1 == N) ? 0 : b + 1, bm = (b == 0) ? N - 1 : b - 1;
     const int64_t col = b * N + c;
     const int64_t jp = ((a0 == 0) ? N - 1 : a0 - 1) * NN + col, jc = a0 * NN + col;
+    const int64_t nbr[4] = {bp * N + c, bm * N + c, b * N + cp, b * N + cm};  // in-plane neighbours
     double up = w[jp], uc = w[jc];            // a2.pi
     double vp = w[N3 + jp], vc = w[N3 + jc];  // a1.pi
     for (int64_t a = a0; a < a1; ++a) {
       const int64_t ap = (a + 1 == N) ? 0 : a + 1;
       const int64_t row = a * NN, i = row + col, jn = ap * NN + col;
-      const int64_t j0 = row + bp * N + c, j1 = row + bm * N + c, j2 = row + b * N + cp, j3 = row + b * N + cm;
+      // every load of the iteration first, so that they are in flight together
       const double un = w[jn], vn = w[N3 + jn];
-      const double xphi = x[i];
-      const double y1 = xphi + k.dh * x[N3 + i];
-      const double nb2 = un + up + w[j0] + w[j1] + w[j2] + w[j3];
+      const double xphi = x[i], xpi = x[N3 + i], lphi = lp[i], sphi = s[i], spi = s[N3 + i];
+      double qu[4], qv[4];
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        qu[d] = w[row + nbr[d]];
+        qv[d] = w[N3 + row + nbr[d]];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      const double y1 = xphi + k.dh * xpi;
+      const double nb2 = un + up + qu[0] + qu[1] + qu[2] + qu[3];
       const double yb2phi = jt_phi(c_m, lam, y1, k.r2.inv, uc, nb2);
-      const double nb1 = vn + vp + w[N3 + j0] + w[N3 + j1] + w[N3 + j2] + w[N3 + j3];
+      const double nb1 = vn + vp + qv[0] + qv[1] + qv[2] + qv[3];
       const double outphi = jt_phi(c_m, lam, xphi, k.r1.inv, vc, nb1);
-      const double outpi = (k.dt6 * lp[i] + k.dh * yb2phi) + k.r1.c_h * vc;
-      lo[i] = (s[i] + yb2phi) + outphi;
-      lo[N3 + i] = s[N3 + i] + outpi;
+      const double outpi = (k.dt6 * lphi + k.dh * yb2phi) + k.r1.c_h * vc;
+      lo[i] = (sphi + yb2phi) + outphi;
+      lo[N3 + i] = spi + outpi;
       dm1 += (-k.two_m * xphi) * vc;
       up = uc; uc = un;
       vp = vc; vc = vn;
