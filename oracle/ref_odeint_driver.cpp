@@ -12,10 +12,13 @@
 //   in_state.bin: 2N^3 little-endian doubles. Writes out_dir/state.bin (final
 //   state), out_dir/times.bin (every time the observer was called) and
 //   out_dir/cosmo.bin (a, H, eta at 16 sample times t0 + k*(t_end-t0)/15).
+//   Prints the number of steps on stdout and "elapsed_s <seconds>" of the
+//   integrate_const call (std::chrono::steady_clock) on stderr.
 #include <boost/numeric/odeint.hpp>
 #include <boost/numeric/odeint/external/eigen/eigen.hpp>
 
 #include <Eigen/Dense>
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -75,7 +78,10 @@ int main(int argc, char **argv) {
   System sys{N, L, m, lambda, (H1 == 0.0) ? StaticEOSCosmology() : StaticEOSCosmology(a1, H1, t1, 1.0)};
   std::vector<double> times;
   auto stepper = runge_kutta4_classic<Eigen::VectorXd, double, Eigen::VectorXd, double>();
+  const auto c0 = std::chrono::steady_clock::now();
   size_t steps = integrate_const(stepper, sys, x, t0, t_end, dt, Recorder{&times});
+  const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
+  fprintf(stderr, "elapsed_s %.6f\n", el);
   std::vector<double> cz;
   for (int k = 0; k < 16; ++k) {
     const double t = t0 + k * (t_end - t0) / 15.0;
