@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 #include "cf_common.hpp"
@@ -57,7 +58,16 @@ constexpr int RED_BLOCKS = 1024;
 constexpr int BIN_LDS = 4096;  // doubles of the step-A tile (32 KB: several blocks per CU)
 constexpr int BIN_CQ = 9;      // 64-column chunks of a row: H = N/2 + 1 <= 513
 
-template <bool MODE>
+// CQ: 64-column chunks per row (CQ * 64 >= H). Each wavefront keeps bin_group rows
+// of its class in flight (their indices read together, then every column load of
+// the group issued before the group is summed, in row order: the same bits as a
+// row-by-row sum); the group holds at most ~96 VGPRs of loaded values.
+constexpr int bin_group(bool mode, int cq) {
+  const int g = 24 / (cq * (mode ? 2 : 1));
+  return g < 1 ? 1 : (g > 4 ? 4 : g);
+}
+
+template <bool MODE, int CQ>
 __global__ __launch_bounds__(256) void k_bin_rows(const double2 *__restrict__ fk, const double2 *__restrict__ dk,
                                                   const int32_t *__restrict__ roff, const int32_t *__restrict__ rows,
                                                   int64_t H, int64_t nr, int lrb, double m2, double kir2, double a2,
@@ -70,35 +80,55 @@ __global__ __launch_bounds__(256) void k_bin_rows(const double2 *__restrict__ fk
     const int64_t r = r0 + rl;
     if (r >= nr) break;
     const int32_t beg = roff[r], end = roff[r + 1];
-    // lane holds columns c = lane + 64 q; a whole row is loaded at once (independent
-    // loads in flight), rows of the class summed in order
-    double acc[BIN_CQ];
-    double denom[BIN_CQ];
+    // lane holds columns c = lane + 64 q
+    double acc[CQ];
+    double denom[CQ];
 #pragma unroll
-    for (int q = 0; q < BIN_CQ; ++q) {
+    for (int q = 0; q < CQ; ++q) {
       acc[q] = 0.0;
       const int64_t c = lane + 64 * q;
       // fdm3d.cpp:64: m*m + (2*pi/L)*(2*pi/L)*s/(a_t*a_t)
       denom[q] = MODE ? m2 + (kir2 * (double)(r + c * c)) / a2 : 1.0;
     }
-    for (int32_t j = beg; j < end; ++j) {
+    auto add = [&](int q, double2 u, double2 w) {
+      const int64_t c = lane + 64 * q;
+      double p = u.x * u.x + u.y * u.y;
+      if constexpr (MODE) p = p + (w.x * w.x + w.y * w.y) / denom[q];
+      acc[q] += (c == 0 || c == H - 1) ? p : 2.0 * p;
+    };
+    const double2 zero = make_double2(0.0, 0.0);
+    constexpr int BIN_G = bin_group(MODE, CQ);
+    int32_t j = beg;
+    for (; j + BIN_G <= end; j += BIN_G) {
+      int64_t base[BIN_G];
+#pragma unroll
+      for (int g = 0; g < BIN_G; ++g) base[g] = (int64_t)rows[j + g] * H;
+      double2 u[BIN_G][CQ], w[BIN_G][CQ];
+#pragma unroll
+      for (int g = 0; g < BIN_G; ++g)
+#pragma unroll
+        for (int q = 0; q < CQ; ++q) {
+          const int64_t c = lane + 64 * q;
+          u[g][q] = c < H ? fk[base[g] + c] : zero;
+          if constexpr (MODE) w[g][q] = c < H ? dk[base[g] + c] : zero;
+          else w[g][q] = zero;
+        }
+#pragma unroll
+      for (int g = 0; g < BIN_G; ++g)
+#pragma unroll
+        for (int q = 0; q < CQ; ++q)
+          if (lane + 64 * q < H) add(q, u[g][q], w[g][q]);
+    }
+    for (; j < end; ++j) {
       const int64_t base = (int64_t)rows[j] * H;
 #pragma unroll
-      for (int q = 0; q < BIN_CQ; ++q) {
+      for (int q = 0; q < CQ; ++q) {
         const int64_t c = lane + 64 * q;
-        if (c < H) {
-          const double2 u = fk[base + c];
-          double p = u.x * u.x + u.y * u.y;
-          if constexpr (MODE) {
-            const double2 w = dk[base + c];
-            p = p + (w.x * w.x + w.y * w.y) / denom[q];
-          }
-          acc[q] += (c == 0 || c == H - 1) ? p : 2.0 * p;
-        }
+        if (c < H) add(q, fk[base + c], MODE ? dk[base + c] : zero);
       }
     }
 #pragma unroll
-    for (int q = 0; q < BIN_CQ; ++q) {
+    for (int q = 0; q < CQ; ++q) {
       const int64_t c = lane + 64 * q;
       if (c < H) tile[rl * H + c] = acc[q];
     }
@@ -396,12 +426,22 @@ static cf_status bin_launch(cf_spectrum_plan *p, const double *fk, const double 
   int lrb = 0;  // RB = 2^lrb row classes per block: the largest power of two with RB * H <= BIN_LDS
   while ((2 << lrb) * p->H <= BIN_LDS && lrb < 8) ++lrb;
   const dim3 gA((unsigned)cdiv(p->nr, (int64_t)1 << lrb)), b256(256);
-  if (dk)
-    hipLaunchKernelGGL(k_bin_rows<true>, gA, b256, 0, st, (const double2 *)fk, (const double2 *)dk, p->roff, p->rows,
-                       p->H, p->nr, lrb, m2, kir2, a2, p->rbuf);
-  else
-    hipLaunchKernelGGL(k_bin_rows<false>, gA, b256, 0, st, (const double2 *)fk, (const double2 *)nullptr, p->roff,
-                       p->rows, p->H, p->nr, lrb, m2, kir2, a2, p->rbuf);
+  auto rows_kernel = [&](auto mode, auto cq) {
+    hipLaunchKernelGGL((k_bin_rows<decltype(mode)::value, decltype(cq)::value>), gA, b256, 0, st,
+                       (const double2 *)fk, (const double2 *)dk, p->roff, p->rows, p->H, p->nr, lrb, m2, kir2, a2,
+                       p->rbuf);
+  };
+  auto by_cq = [&](auto mode) {
+    const int64_t cq = cdiv(p->H, 64);  // H <= 64 * BIN_CQ (checked above)
+    if (cq <= 1) rows_kernel(mode, std::integral_constant<int, 1>());
+    else if (cq <= 2) rows_kernel(mode, std::integral_constant<int, 2>());
+    else if (cq <= 3) rows_kernel(mode, std::integral_constant<int, 3>());
+    else if (cq <= 4) rows_kernel(mode, std::integral_constant<int, 4>());
+    else if (cq <= 5) rows_kernel(mode, std::integral_constant<int, 5>());
+    else rows_kernel(mode, std::integral_constant<int, BIN_CQ>());
+  };
+  if (dk) by_cq(std::true_type());
+  else by_cq(std::false_type());
   CF_LAUNCHED();
   hipLaunchKernelGGL(k_bin_cols, dim3((unsigned)cdiv(p->nb, 256)), b256, 0, st, p->rbuf, p->H, p->nr, p->nb, bins);
   CF_LAUNCHED();

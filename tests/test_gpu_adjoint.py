@@ -95,6 +95,33 @@ def test_step_adjoint_vs_torch(cfl, O, lam):
     assert abs(host(part).sum() - gm.item()) <= 1e-12 * abs(gm.item())
 
 
+@pytest.mark.parametrize("N,lam", [(70, 0.25), (34, 0.0)])
+def test_step_adjoint_partial_tiles_vs_torch(cfl, O, N, lam):
+    """One step back at N = 70 (a full and a 6-column c tile, 18 b tiles of 4 with a
+    2-row last one) and N = 34 (one c tile with 30 idle lanes): the fused backward
+    kernels skip the columns outside the lattice and wrap the neighbours; the work
+    buffer is exactly the documented 3 x 2N^3 doubles."""
+    import ctypes as C
+    print(f"case step adjoint N={N} lam={lam}", flush=True)
+    L, m, t, dt = float(N), 1.2, 1.4, 0.07
+    cosmo_o = O.Cosmo.radiation(**FRW)
+    rng = np.random.default_rng(N)
+    x = np.concatenate([0.5 * rng.standard_normal(N ** 3), 0.3 * rng.standard_normal(N ** 3)])
+    lamv = rng.standard_normal(2 * N ** 3)
+    p = cfl.kg_params(N, L, m, lam, cfl.radiation(**FRW))
+    ld = torch.from_numpy(lamv.copy()).cuda()
+    part = torch.zeros(4 * 512, dtype=torch.float64, device="cuda")
+    work = torch.empty(3 * 2 * N ** 3, dtype=torch.float64, device="cuda")
+    cfl.check(cfl.lib.cf_rk4_step_adjoint_f64(C.byref(p), cfl._ptr(torch.from_numpy(x).cuda()), t, dt, cfl._ptr(ld),
+                                              cfl._ptr(part), cfl._ptr(work), cfl._stream()))
+    xt = torch.tensor(x, requires_grad=True)
+    mt = torch.tensor(m, dtype=torch.float64, requires_grad=True)
+    y = t_step(xt, N, L, mt, lam, cosmo_o, t, dt, O)
+    gx, gm = torch.autograd.grad(y, (xt, mt), grad_outputs=torch.from_numpy(lamv))
+    assert rel(host(ld), gx.numpy()) < 1e-13
+    assert abs(host(part).sum() - gm.item()) <= 1e-12 * abs(gm.item())
+
+
 @pytest.mark.parametrize("n_steps,ckpt,lam", [(24, 8, 0.0), (23, 5, 0.2)])
 def test_full_chain_vs_torch(cfl, O, n_steps, ckpt, lam):
     """Checkpointed forward + backward through n_steps equals torch autograd of the
