@@ -189,8 +189,8 @@ def observer_timing(cfl, plan, x, N, L, m, cosmo, t, stream, reps=5):
     def observe():
         plan.mode_power_spectrum(x, L, m, a_t)
         cfl.energy_density_f64(N, x, rho, m, 0.0, cfl.inv_ah_sqr(N, L, a_t))
-        plan.power_spectrum(rho)
         cfl.axis_average(N, rho, avg)
+        plan.power_spectrum(rho, consume=True)  # the observer is done with rho
 
     observe()
     torch.cuda.synchronize()

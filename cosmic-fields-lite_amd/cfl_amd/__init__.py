@@ -210,9 +210,12 @@ class SpectrumPlan:
         check(lib.cf_fft_z2d(self.h, _ptr(fk), _ptr(out), _stream(stream)))
         return out
 
-    def power_spectrum(self, f, out=None, stream=None):
+    def power_spectrum(self, f, out=None, stream=None, consume=False):
+        """Bins of |f_k|^2. consume=True hands f to the transform without the copy
+        that preserves it: f's contents are undefined afterwards."""
         out = self._bins(out)
-        check(lib.cf_power_spectrum(self.h, _ptr(f), _ptr(out), _stream(stream)))
+        fn = lib.cf_power_spectrum_consume if consume else lib.cf_power_spectrum
+        check(fn(self.h, _ptr(f), _ptr(out), _stream(stream)))
         return out
 
     def mode_power_spectrum(self, state, L, m, a_t, out=None, stream=None):

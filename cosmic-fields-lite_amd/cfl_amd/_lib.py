@@ -130,6 +130,7 @@ _SIGS = [
     ("cf_fft_d2z", st, [vp, vp, vp, vp]),
     ("cf_fft_z2d", st, [vp, vp, vp, vp]),
     ("cf_power_spectrum", st, [vp, vp, vp, vp]),
+    ("cf_power_spectrum_consume", st, [vp, vp, vp, vp]),
     ("cf_mode_power_spectrum", st, [vp, vp, dbl, dbl, dbl, vp, vp]),
     ("cf_bin_power", st, [vp, vp, vp, vp]),
     ("cf_bin_mode_power", st, [vp, vp, vp, dbl, dbl, dbl, vp, vp]),

@@ -569,7 +569,7 @@ cf_status cf_ensemble_density_spectrum(cf_ensemble *e, int r, double t, double *
   double *rho = e->scratch + 2 * n3;  // after the joined state (scratch holds 3 N_max^3)
   const double a_t = cf_cosmo_a(&e->cosmo, t);
   CF_TRY(cf_energy_density_f64(R.N, x, rho, e->m, e->lambda, cf_inv_ah_sqr(R.N, R.L, a_t), stream));
-  return cf_power_spectrum(pl, rho, bins_dev, stream);
+  return cf_power_spectrum_consume(pl, rho, bins_dev, stream);  // rho is this call's scratch
 }
 
 }  // extern "C"

@@ -469,6 +469,13 @@ cf_status cf_power_spectrum(cf_spectrum_plan *p, const double *f, double *bins, 
   return cf_bin_power(p, p->cbuf0, bins, stream);
 }
 
+cf_status cf_power_spectrum_consume(cf_spectrum_plan *p, double *f, double *bins, cf_stream stream) {
+  CF_TRY(plan_check(p));
+  CF_REQUIRE(f && bins, "cf_power_spectrum_consume: null buffer");
+  CF_TRY(d2z_exec(p, f, p->cbuf0, S(stream)));
+  return cf_bin_power(p, p->cbuf0, bins, stream);
+}
+
 cf_status cf_mode_power_spectrum(cf_spectrum_plan *p, const double *state, double L, double m,
                                  double a_t, double *bins, cf_stream stream) {
   CF_TRY(plan_check(p));

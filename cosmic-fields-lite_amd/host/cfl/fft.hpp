@@ -67,6 +67,16 @@ inline hip_vector<double> compute_power_spectrum(const long long N, const hip_ve
   return out;
 }
 
+// The same bins from a field the caller is done with: f is the transform's input
+// without the copy that preserves it, and its contents are undefined afterwards.
+inline hip_vector<double> compute_power_spectrum_consume(const long long N, hip_vector<double> &f,
+                                                         hipfftWrapper &fft) {
+  hip_vector<double> out(spectrum_size(N));
+  check(cf_power_spectrum_consume(fft.plan(), f.data(), out.data(), default_stream()),
+        "compute_power_spectrum_consume");
+  return out;
+}
+
 // compute_mode_power_spectrum (fdm3d.cpp:36-72 / fdm3d_cuda.cu:144-159)
 inline hip_vector<double> compute_mode_power_spectrum(const long long N, const double L, const double m,
                                                       const double a_t, const hip_vector<double> &state,

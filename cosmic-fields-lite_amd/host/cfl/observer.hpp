@@ -91,16 +91,10 @@ struct ConstIntervalObserver {
     }
     if constexpr (save_density_spectrum || save_density) {
       hip_vector<double> rho = Equation::compute_energy_density(workspace, t);
-      if constexpr (save_density_spectrum) {
-        hip_vector<double> s = compute_power_spectrum(N, rho, workspace.fft_wrapper);
-        write_vector_to_filename_template(to_host(s), dir + "rho_spectrum_%d.dat", idx);
-        if constexpr (save_radial)
-          write_vector_to_filename_template(to_host(compute_radial_spectrum(N, s, workspace.fft_wrapper)),
-                                            dir + "rho_radial_%d.dat", idx);
-      }
       if constexpr (save_density) {
         // slice at a = 0 and the average over a (observer.hpp:110-119), both
-        // taken on the device: 2 N^2 values cross PCIe instead of N^3
+        // taken on the device: 2 N^2 values cross PCIe instead of N^3. Taken
+        // before the spectrum, whose transform consumes rho.
         const std::size_t n2 = (std::size_t)(N * N);
         std::vector<double> slice(n2), avg(n2);
         hip_vector<double> avg_d(n2);
@@ -109,6 +103,13 @@ struct ConstIntervalObserver {
         check(cf_d2h(avg.data(), avg_d.data(), n2 * sizeof(double), default_stream()), "rho axis average");
         write_vector_to_filename_template(slice, dir + "rho_slice_%d.dat", idx);
         write_vector_to_filename_template(avg, dir + "rho_axis_average_%d.dat", idx);
+      }
+      if constexpr (save_density_spectrum) {
+        hip_vector<double> s = compute_power_spectrum_consume(N, rho, workspace.fft_wrapper);
+        write_vector_to_filename_template(to_host(s), dir + "rho_spectrum_%d.dat", idx);
+        if constexpr (save_radial)
+          write_vector_to_filename_template(to_host(compute_radial_spectrum(N, s, workspace.fft_wrapper)),
+                                            dir + "rho_radial_%d.dat", idx);
       }
     }
   }

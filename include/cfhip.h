@@ -306,6 +306,12 @@ cf_status cf_fft_z2d(cf_spectrum_plan *plan, double *in_dev, double *out_dev, cf
  * bins_dev[s] = sum over the full k-cube with |n|^2 = s of |f_k|^2. Deterministic. */
 cf_status cf_power_spectrum(cf_spectrum_plan *plan, const double *f_dev, double *bins_dev,
                             cf_stream stream);
+/* The same bins, with f_dev handed to the transform as its input without the copy
+ * that preserves it: f_dev's contents are undefined afterwards (rocFFT may use an
+ * out-of-place transform's input as scratch). For a caller that owns f_dev and is
+ * done with it, e.g. the observer's energy density once its slice and axis average
+ * are taken (observer.hpp:102-119): saves an N^3 copy per spectrum. */
+cf_status cf_power_spectrum_consume(cf_spectrum_plan *plan, double *f_dev, double *bins_dev, cf_stream stream);
 /* compute_mode_power_spectrum (fdm3d.cpp:36-72; fdm3d_cuda.cu:144-159):
  * bins_dev[s] = sum |phi_k|^2 + |dphi_k|^2 / (m^2 + (2 pi/L)^2 s / a^2). */
 cf_status cf_mode_power_spectrum(cf_spectrum_plan *plan, const double *state_dev, double L, double m,

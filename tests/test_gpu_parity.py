@@ -166,6 +166,8 @@ def test_power_spectrum(cfl, O, N):
     assert_bitwise(host(fd), f, "input preserved")
     # deterministic binning: a second call gives identical bits
     assert_bitwise(host(plan.power_spectrum(fd)), g, "repeat")
+    # the consuming form (no input copy) gives the same bins
+    assert_bitwise(host(plan.power_spectrum(fd, consume=True)), g, "consume")
 
 
 @pytest.mark.parametrize("N", [6, 8, 64, 96])
