@@ -10,8 +10,8 @@
 // counterpart (the lambda*phi^4 term), the formula of the reference's CUDA
 // kernel is used and the citation says so.
 //
-// Parity pins for each function are listed in oracle/README.md and checked by
-// tests/test_oracle_*.py.
+// Each function's parity pins (golden vectors, the reference's own integrator
+// build, closed forms and planted mistakes) are the tests/test_oracle_*.py files.
 
 #include <algorithm>
 #include <cmath>
