@@ -336,7 +336,11 @@ cf_status cf_loss_spectrum_adjoint(cf_spectrum_plan *plan, const cf_kg_params *p
   CF_TRY(cf_fft_d2z(plan, rho, fk, stream));
   double *bins = nullptr, *part = nullptr;
   CF_HIP(hipMallocAsync((void **)&bins, nb * sizeof(double), st));
-  CF_HIP(hipMallocAsync((void **)&part, ADJ_BLOCKS * sizeof(double), st));
+  if (hipMallocAsync((void **)&part, ADJ_BLOCKS * sizeof(double), st) != hipSuccess) {
+    (void)hipFreeAsync(bins, st);
+    set_error("cf_loss_spectrum_adjoint: device allocation failed");
+    return CF_E_HIP;
+  }
   cf_status rc = cf_bin_power(plan, fk, bins, stream);
   std::vector<double> hb((size_t)nb), hw((size_t)nb), hp(ADJ_BLOCKS);
   if (rc == CF_OK && (hipMemcpyAsync(hb.data(), bins, nb * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -355,9 +359,9 @@ cf_status cf_loss_spectrum_adjoint(cf_spectrum_plan *plan, const cf_kg_params *p
   }
   if (rc == CF_OK && hipMemcpyAsync(hp.data(), part, ADJ_BLOCKS * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
     rc = CF_E_HIP;
+  (void)hipFreeAsync(bins, st);
+  (void)hipFreeAsync(part, st);
   hipError_t se = hipStreamSynchronize(st);
-  (void)hipFree(bins);
-  (void)hipFree(part);
   if (rc != CF_OK) {
     set_error("cf_loss_spectrum_adjoint: device operation failed");
     return rc;
