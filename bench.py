@@ -419,8 +419,9 @@ def main():
         ensemble_workload(args)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "see DESIGN.md for how the lattice would shard", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the "
+              "decomposition over several GPUs (slabs along a, one ghost phi-plane per stage) "
+              "is designed, not built", file=sys.stderr)
         sys.exit(2)
     import torch
     import cfl_amd as cfl
