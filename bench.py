@@ -396,30 +396,30 @@ def main():
     args = parse()
     if args.workload == "variants":
         if args.gpus != 1:
-            print("bench.py: this build runs on one GPU only", file=sys.stderr)
+            print("bench.py: this library runs on one GPU only", file=sys.stderr)
             sys.exit(2)
         variants_workload(args)
         return
     if args.workload == "adjoint":
         if args.gpus != 1:
-            print("bench.py: this build runs on one GPU only", file=sys.stderr)
+            print("bench.py: this library runs on one GPU only", file=sys.stderr)
             sys.exit(2)
         adjoint_workload(args)
         return
     if args.workload == "precision":
         if args.gpus != 1:
-            print("bench.py: this build runs on one GPU only", file=sys.stderr)
+            print("bench.py: this library runs on one GPU only", file=sys.stderr)
             sys.exit(2)
         precision_workload(args)
         return
     if args.workload == "ensemble":
         if args.gpus != 1:
-            print("bench.py: this build runs on one GPU only", file=sys.stderr)
+            print("bench.py: this library runs on one GPU only", file=sys.stderr)
             sys.exit(2)
         ensemble_workload(args)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the "
+        print(f"bench.py: this library runs on one GPU only (asked for --gpus {args.gpus}); the "
               "decomposition over several GPUs (slabs along a, one ghost phi-plane per stage) "
               "is designed, not built", file=sys.stderr)
         sys.exit(2)

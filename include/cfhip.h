@@ -30,7 +30,7 @@ typedef int32_t cf_status;
 #define CF_E_HIP 2         /* HIP runtime error */
 #define CF_E_FFT 3         /* hipFFT error */
 #define CF_E_NOMEM 4       /* device or host allocation failed */
-#define CF_E_UNSUPPORTED 5 /* valid request this build does not implement */
+#define CF_E_UNSUPPORTED 5 /* valid request this library does not implement */
 
 typedef void *cf_stream; /* a hipStream_t; NULL is the default stream */
 typedef struct cf_spectrum_plan cf_spectrum_plan; /* FFT plans + bin table of one N (below) */
