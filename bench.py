@@ -394,35 +394,23 @@ def variants_workload(args):
 
 def main():
     args = parse()
-    if args.workload == "variants":
-        if args.gpus != 1:
-            print("bench.py: this library runs on one GPU only", file=sys.stderr)
-            sys.exit(2)
-        variants_workload(args)
-        return
-    if args.workload == "adjoint":
-        if args.gpus != 1:
-            print("bench.py: this library runs on one GPU only", file=sys.stderr)
-            sys.exit(2)
-        adjoint_workload(args)
-        return
-    if args.workload == "precision":
-        if args.gpus != 1:
-            print("bench.py: this library runs on one GPU only", file=sys.stderr)
-            sys.exit(2)
-        precision_workload(args)
-        return
-    if args.workload == "ensemble":
-        if args.gpus != 1:
-            print("bench.py: this library runs on one GPU only", file=sys.stderr)
-            sys.exit(2)
-        ensemble_workload(args)
-        return
     if args.gpus != 1:
         print(f"bench.py: this library runs on one GPU only (asked for --gpus {args.gpus}); the "
               "decomposition over several GPUs (slabs along a, one ghost phi-plane per stage) "
               "is designed, not built", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "variants":
+        variants_workload(args)
+        return
+    if args.workload == "adjoint":
+        adjoint_workload(args)
+        return
+    if args.workload == "precision":
+        precision_workload(args)
+        return
+    if args.workload == "ensemble":
+        ensemble_workload(args)
+        return
     import torch
     import cfl_amd as cfl
 
