@@ -74,6 +74,16 @@ def test_sizes(lib, O):
     assert L.cf_rk4_work_bytes(512, 0) == 3 * 2 * 512 ** 3 * 8
     assert L.cf_rk4_work_bytes(512, 1) == 3 * 2 * 512 ** 3 * 4
     assert L.cf_rk4_work_bytes(7, 0) == 0  # odd N unsupported
+    # temporally blocked steps ping-pong with one more state (FP32-comp: hi and lo)
+    assert L.cf_rk4_tb_work_bytes(512, 0) == 2 * 512 ** 3 * 8
+    assert L.cf_rk4_tb_work_bytes(512, 1) == 2 * 2 * 512 ** 3 * 4
+    # adjoint: checkpoints + one segment + step work (2) + lambda (2) + forward work
+    # (3) state vectors, and 4 dL/dm partials per block (512 blocks) per step plus one
+    v = 2 * 128 ** 3 * 8
+    for n_steps, ckpt in ((1024, 32), (23, 5), (0, 4)):
+        nck = -(-n_steps // ckpt)
+        assert L.cf_adjoint_work_bytes(128, n_steps, ckpt) == (nck + ckpt + 7) * v + (4 * n_steps + 1) * 512 * 8
+    assert L.cf_adjoint_work_bytes(128, 10, 0) == 0 and L.cf_adjoint_work_bytes(7, 10, 2) == 0
 
 
 def test_bad_arguments_rejected_before_device_work(lib):
