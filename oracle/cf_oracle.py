@@ -23,6 +23,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+# Library this module binds. Tests that plant a mistake in a copy of the oracle
+# load that copy into a separate instance of this module by changing the path.
+_PATH = os.path.join(_HERE, "libcforacle.so")
 
 DBL_EPS = np.finfo(np.float64).eps
 
@@ -55,7 +58,7 @@ _fp = C.POINTER(C.c_float)
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "libcforacle.so")
+        path = _PATH
         if not os.path.exists(path):
             raise RuntimeError("oracle not built: run `make -C oracle`")
         L = C.CDLL(path)
