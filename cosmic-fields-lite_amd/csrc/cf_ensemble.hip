@@ -160,6 +160,8 @@ struct cf_ensemble {
   int32_t *d_unit_off = nullptr;
   EnsUnit *d_units = nullptr;
   EnsStepCoef *d_steps = nullptr;
+  EnsStepCoef *h_steps = nullptr;  // pinned staging of the step table
+  hipEvent_t steps_ev = nullptr;   // recorded after the table's last upload
   int64_t steps_cap = 0;
   int parity = 0;
   hipStream_t st = nullptr;
@@ -182,14 +184,21 @@ static void ens_free_plan(cf_ensemble *e) {
   e->d_units = nullptr;
 }
 
+// The graph is dropped whenever the tables it reads (units, step scalars) are
+// rewritten; a replay still in flight on the ensemble's stream ends first.
 static void ens_drop_graph(cf_ensemble *e) {
-  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->exec) {
+    (void)hipStreamSynchronize(e->st);
+    (void)hipGraphExecDestroy(e->exec);
+  }
   e->exec = nullptr;
   e->g_k0 = -1;
 }
 
-// Build the unit table for the current chunks and grid, LPT-balanced.
+// Build the unit table for the current chunks and grid, LPT-balanced. Launches
+// still in flight read the old table, so they end before it is freed.
 static cf_status ens_build(cf_ensemble *e) {
+  CF_HIP(hipStreamSynchronize(e->st));
   std::vector<EnsUnit> units;
   std::vector<int> cost;
   for (int r = 0; r < e->n; ++r) {
@@ -252,15 +261,27 @@ static cf_status ens_build(cf_ensemble *e) {
   return CF_OK;
 }
 
-// Step table for steps k0 .. k0+n-1 at t_k = t0 + k*dt.
+// Step table for steps k0 .. k0+n-1 at t_k = t0 + k*dt. The upload is queued on
+// the ensemble's stream, behind every launch of earlier runs that still reads the
+// previous table, from a pinned staging buffer that is rewritten only after its
+// previous upload has completed. Runs may therefore follow one another without a
+// host synchronisation. A captured graph reads the table, so it is dropped.
 static cf_status ens_fill_steps(cf_ensemble *e, double t0, int64_t k0, int64_t n, double dt) {
+  ens_drop_graph(e);
   if (n > e->steps_cap) {
+    CF_HIP(hipStreamSynchronize(e->st));  // launches in flight read the old buffers
     if (e->d_steps) CF_HIP(hipFree(e->d_steps));
+    if (e->h_steps) CF_HIP(hipHostFree(e->h_steps));
     e->d_steps = nullptr;
+    e->h_steps = nullptr;
+    e->steps_cap = 0;
     CF_HIP(hipMalloc((void **)&e->d_steps, (size_t)n * sizeof(EnsStepCoef)));
+    CF_HIP(hipHostMalloc((void **)&e->h_steps, (size_t)n * sizeof(EnsStepCoef), hipHostMallocDefault));
     e->steps_cap = n;
+  } else {
+    CF_HIP(hipEventSynchronize(e->steps_ev));  // the staging buffer's previous upload is done
   }
-  std::vector<EnsStepCoef> h((size_t)n);
+  EnsStepCoef *h = e->h_steps;
   for (int64_t s = 0; s < n; ++s) {
     const double t = t0 + static_cast<double>(k0 + s) * dt;
     const double dh = dt / 2.0, th = t + dh;
@@ -270,7 +291,8 @@ static cf_status ens_fill_steps(cf_ensemble *e, double t0, int64_t k0, int64_t n
       h[s].H[q] = cf_cosmo_H(&e->cosmo, ts[q]);
     }
   }
-  CF_HIP(hipMemcpy(e->d_steps, h.data(), h.size() * sizeof(EnsStepCoef), hipMemcpyHostToDevice));
+  CF_HIP(hipMemcpyAsync(e->d_steps, h, (size_t)n * sizeof(EnsStepCoef), hipMemcpyHostToDevice, e->st));
+  CF_HIP(hipEventRecord(e->steps_ev, e->st));
   return CF_OK;
 }
 
@@ -355,7 +377,10 @@ cf_status cf_ensemble_destroy(cf_ensemble *e) {
   if (!e) return CF_OK;
   ens_drop_graph(e);
   ens_free_plan(e);
+  if (e->st) (void)hipStreamSynchronize(e->st);
   if (e->d_steps) (void)hipFree(e->d_steps);
+  if (e->h_steps) (void)hipHostFree(e->h_steps);
+  if (e->steps_ev) (void)hipEventDestroy(e->steps_ev);
   if (e->d_reals) (void)hipFree(e->d_reals);
   for (void *p : e->allocs) (void)hipFree(p);
   for (auto &kv : e->plans) cf_spectrum_plan_destroy(kv.second);
@@ -383,7 +408,8 @@ cf_status cf_ensemble_create(int n, const int64_t *N, const double *L, double m,
     cf_ensemble_destroy(e);
     return s;
   };
-  if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&e->steps_ev, hipEventDisableTiming) != hipSuccess) {
     set_error("cf_ensemble_create: stream");
     return fail(CF_E_HIP);
   }
@@ -535,6 +561,7 @@ cf_status cf_ensemble_run(cf_ensemble *e, double t0, int64_t k0, int64_t n_steps
       cf_status rc = CF_OK;
       for (int64_t s = 0; s < n_steps && rc == CF_OK; ++s) rc = ens_launch(e, s, e->parity ^ (int)(s & 1), dt, e->st);
       hipError_t ce = hipStreamEndCapture(e->st, &g);
+      if (ce == hipSuccess && rc != CF_OK) (void)hipGraphDestroy(g);
       if (rc != CF_OK) return rc;
       CF_HIP(ce);
       hipError_t ie = hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0);

@@ -156,3 +156,42 @@ def test_flat_grouped_xcd_order(cfl):
         assert_bitwise(host(ghi), host(hi), f"hi r={r}")
         assert_bitwise(host(glo), host(lo), f"lo r={r}")
     ens.close()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_segments_without_host_sync(cfl, use_graph):
+    """Two runs of 3 steps issued back to back (no synchronisation in between: the
+    second run's step table is uploaded while the first run may still be reading its
+    own) equal one run of 6 steps bit for bit."""
+    print(f"case segments use_graph={use_graph}", flush=True)
+    a = make(cfl, 1)
+    b = make(cfl, 1)
+    torch.cuda.synchronize()
+    a.run(1.0, 0, 3, 0.05, use_graph=use_graph)
+    a.run(1.0, 3, 3, 0.05, use_graph=use_graph)
+    b.run(1.0, 0, 6, 0.05, use_graph=False)
+    torch.cuda.synchronize()
+    for r in range(a.n):
+        for x, y in zip(raw_state(cfl, a, r), raw_state(cfl, b, r)):
+            assert_bitwise(host(x), host(y), f"segments r={r}")
+    a.close()
+    b.close()
+
+
+def test_graph_not_replayed_after_eager_run(cfl):
+    """A captured graph (steps k0 = 0..3), then an eager run of other steps (k0 = 4..7,
+    which rewrites the step table), then the first call again: the last call must run
+    the steps of k0 = 0..3, not replay a graph that reads the rewritten table."""
+    a = make(cfl, 1)
+    b = make(cfl, 1)
+    a.run(1.0, 0, 4, 0.05, use_graph=True)
+    a.run(1.0, 4, 4, 0.05, use_graph=False)
+    a.run(1.0, 0, 4, 0.05, use_graph=True)
+    for k0 in (0, 4, 0):
+        b.run(1.0, k0, 4, 0.05, use_graph=False)
+    torch.cuda.synchronize()
+    for r in range(a.n):
+        for x, y in zip(raw_state(cfl, a, r), raw_state(cfl, b, r)):
+            assert_bitwise(host(x), host(y), f"graph after eager r={r}")
+    a.close()
+    b.close()
