@@ -66,6 +66,23 @@ def traffic_per_launch(kernel, N):
     return None
 
 
+VALU_PROFILE = os.path.join(ROOT, "profiles", "rk4_tb_N512_pmc_valu.md")
+
+
+def valu_busy(kernel, N):
+    """VALU-busy fraction of `kernel` at size N from the committed rocprofv3 PMC
+    summary (lines `valu_busy <kernel template> <N> <fraction>`), or None."""
+    if not os.path.exists(VALU_PROFILE):
+        return None
+    key = kernel.split(" (")[0]
+    for line in open(VALU_PROFILE):
+        if line.startswith("valu_busy ") and key in line:
+            parts = line.split()
+            if int(parts[-2]) == N:
+                return float(parts[-1])
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -435,22 +452,29 @@ def main():
     paths = [q for q in args.paths.split(",") if q]
     results = {}
     for mode in paths:
-        if mode.endswith("f64"):
-            x = x0.clone()
-            if mode == "f64":
-                work = cfl.alloc_work(N, cfl.CF_PREC_F64)
-                run = lambda k0, n: cfl.rk4_run_f64(p, x, t0, k0, n, dt, work)  # noqa: E731
+        if mode.startswith("tb_"):
+            # one out-of-place launch per step, ping-ponging between two state
+            # buffers: the timed region holds exactly `steps` kernel launches
+            if mode == "tb_f64":
+                bufs = [(x0.clone(),), (torch.empty_like(x0),)]
+                step = lambda k, i, o: cfl.rk4_step_tb_f64(p, i[0], o[0], t0 + k * dt, dt, args.chunk)  # noqa: E731
             else:
-                work = cfl.alloc_tb_work(N, cfl.CF_PREC_F64)
-                run = lambda k0, n: cfl.rk4_run_tb_f64(p, x, t0, k0, n, dt, work, args.chunk)  # noqa: E731
+                bufs = [cfl.split_f64(x0), tuple(torch.empty(2 * N ** 3, dtype=torch.float32, device="cuda")
+                                                 for _ in range(2))]
+                step = lambda k, i, o: cfl.rk4_step_tb_f32c(p, i[0], i[1], o[0], o[1], t0 + k * dt, dt,  # noqa: E731
+                                                            args.chunk)
+
+            def run(k0, n, bufs=bufs, step=step):
+                for k in range(k0, k0 + n):
+                    step(k, bufs[k & 1], bufs[(k + 1) & 1])
+        elif mode == "f64":
+            x = x0.clone()
+            work = cfl.alloc_work(N, cfl.CF_PREC_F64)
+            run = lambda k0, n: cfl.rk4_run_f64(p, x, t0, k0, n, dt, work)  # noqa: E731
         else:
             hi, lo = cfl.split_f64(x0)
-            if mode == "f32c":
-                work = cfl.alloc_work(N, cfl.CF_PREC_F32C)
-                run = lambda k0, n: cfl.rk4_run_f32c(p, hi, lo, t0, k0, n, dt, work)  # noqa: E731
-            else:
-                work = cfl.alloc_tb_work(N, cfl.CF_PREC_F32C)
-                run = lambda k0, n: cfl.rk4_run_tb_f32c(p, hi, lo, t0, k0, n, dt, work, args.chunk)  # noqa: E731
+            work = cfl.alloc_work(N, cfl.CF_PREC_F32C)
+            run = lambda k0, n: cfl.rk4_run_f32c(p, hi, lo, t0, k0, n, dt, work)  # noqa: E731
         run(0, args.warmup)
         torch.cuda.synchronize()
         # HIP events on the stream the kernels are launched on (torch's current stream)
@@ -464,26 +488,33 @@ def main():
         ms = ev0.elapsed_time(ev1) / args.steps
         rate = N ** 3 / (ms * 1e-3) / 1e9
         canon = BYTES_PER_SITE_STEP[mode.replace("tb_", "")]
+        kb = DESIGN_BYTES_PER_SITE_STEP[mode]
         results[mode] = {"value": rate, "ms_per_step": ms, "wall_s": wall,
                          "kernel": KERNEL_NAMES[mode],
+                         "kernel_bytes_per_site_step": kb,
+                         "kernel_frac_of_8TBs": kb * rate / HBM_PEAK_GBS,
                          "canonical_bytes_per_site_step": canon,
-                         "kernel_bytes_per_site_step": DESIGN_BYTES_PER_SITE_STEP[mode],
-                         "canonical_frac_of_8TBs": canon * rate / HBM_PEAK_GBS,
-                         "kernel_GBs": DESIGN_BYTES_PER_SITE_STEP[mode] * rate}
+                         "canonical_frac_of_8TBs": canon * rate / HBM_PEAK_GBS}
         print(json.dumps({"path": mode, **results[mode]}), file=sys.stderr, flush=True)
-        del work
+        run = step = bufs = work = None
         torch.cuda.synchronize()
 
     # Headline: run A (FP32-compensated storage), the fastest of its kernels; the
     # FP64 paths (run B) are reported beside it.
     a_modes = [q for q in ("tb_f32c", "f32c") if q in results]
     b_modes = [q for q in ("tb_f64", "f64") if q in results]
-    A = results[max(a_modes, key=lambda q: results[q]["value"])] if a_modes else None
-    head = A if A is not None else max(results.values(), key=lambda r: r["value"])
-    head_mode = [q for q, r in results.items() if r is head][0]
-    canon = BYTES_PER_SITE_STEP[head_mode.replace("tb_", "")]
-    achieved = canon * N ** 3 / (head["ms_per_step"] * 1e-3) / 1e9
+    best_a = max(a_modes, key=lambda q: results[q]["value"]) if a_modes else None
+    best_b = max(b_modes, key=lambda q: results[q]["value"]) if b_modes else None
+    head_mode = best_a if best_a is not None else max(results, key=lambda q: results[q]["value"])
+    head = results[head_mode]
+    # roofline of the headline kernel: the bytes it must move per site-step (state
+    # read once and written once for the temporally blocked step) over its average
+    # launch time (HIP events around `steps` launches on the launch stream)
+    kb = head["kernel_bytes_per_site_step"]
+    achieved = kb * N ** 3 / (head["ms_per_step"] * 1e-3) / 1e9
+    canon = head["canonical_bytes_per_site_step"]
     traffic = traffic_per_launch(head["kernel"], N)
+    valu = valu_busy(head["kernel"], N)
     line = {
         "metric": "GSite*RK4-step/s + % HBM roofline, FP32-comp vs FP64 at N=512^3",
         "value": head["value"],
@@ -506,20 +537,25 @@ def main():
         "roofline": {"bound": "hbm", "kernel": head["kernel"],
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "algorithmic_bytes_per_site_step": canon,
-                     "kernel_bytes_per_site_step": DESIGN_BYTES_PER_SITE_STEP[head_mode],
                      "traffic": traffic,
-                     "traffic_GBs": traffic / (head["ms_per_step"] * 1e-3) / 1e9 if traffic else None,
+                     "algorithmic_bytes_per_site_step": kb,
+                     "traffic_bytes_per_site_step": traffic / N ** 3 if traffic else None,
                      "traffic_frac": traffic / (head["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic
                      else None,
-                     "note": "achieved = canonical per-stage-schedule bytes x sites / launch time; a "
-                             "temporally blocked step moves fewer bytes (kernel_bytes_per_site_step; "
-                             "traffic = rocprofv3 FETCH_SIZE+WRITE_SIZE per launch, calibrated; "
-                             "traffic_GBs = traffic / launch time). The temporally blocked kernel is "
-                             "bound by vector-ALU issue, not HBM (VALU busy ~70-76%, "
-                             "profiles/rk4_tb_N512_pmc_valu.md)"},
+                     "canonical_bytes_per_site_step": canon,
+                     "canonical_frac": canon * head["value"] / HBM_PEAK_GBS,
+                     "limiter": "vector-ALU issue" if head_mode.startswith("tb_") else "hbm",
+                     "valu_busy": valu,
+                     "note": "achieved = the bytes the kernel must move per site-step (read and write the "
+                             "state once) x sites / average launch time; traffic = rocprofv3 FETCH_SIZE + "
+                             "WRITE_SIZE per launch (calibrated, profiles/); canonical_frac uses the "
+                             "one-kernel-per-stage schedule's bytes (152 B FP32-comp, 272 B FP64) and can "
+                             "exceed 1 for a temporally blocked step; valu_busy = SQ_ACTIVE_INST_VALU / "
+                             "(CUs x kernel cycles) from profiles/"},
     }
-    for a, b in (("tb_f32c", "tb_f64"), ("f32c", "f64"), ("tb_f32c", "f64")):
+    if best_a and best_b:
+        line["speedup_f32c_vs_fastest_f64"] = results[best_a]["value"] / results[best_b]["value"]
+    for a, b in (("tb_f32c", "tb_f64"), ("f32c", "f64")):
         if a in results and b in results:
             line[f"speedup_{a}_vs_{b}"] = results[a]["value"] / results[b]["value"]
     if not args.no_cpu:
