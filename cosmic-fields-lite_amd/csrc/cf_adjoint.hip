@@ -326,6 +326,8 @@ cf_status cf_loss_spectrum_adjoint(cf_spectrum_plan *plan, const cf_kg_params *p
                                    double *dldm, cf_stream stream) {
   CF_REQUIRE(plan && p && x && weights_dev && gx_dev && scratch_dev && loss && dldm,
              "cf_loss_spectrum_adjoint: null argument");
+  CF_REQUIRE(plan_size(plan) == p->N, "cf_loss_spectrum_adjoint: plan is for N=%lld, lattice N=%lld",
+             (long long)plan_size(plan), (long long)p->N);
   const int64_t N = p->N, N3 = N * N * N, H = N / 2 + 1, nb = cf_spectrum_bins(N);
   hipStream_t st = S(stream);
   const double a_t = cf_cosmo_a(&p->cosmo, t);
@@ -334,13 +336,8 @@ cf_status cf_loss_spectrum_adjoint(cf_spectrum_plan *plan, const cf_kg_params *p
   CF_TRY(cf_energy_density_f64(N, x, rho, p->m, p->lambda, inv, stream));
   double *fk = gx_dev;  // the half spectrum (2 N^2 H doubles) fits in gx (2N^3) until rho's gradient is formed
   CF_TRY(cf_fft_d2z(plan, rho, fk, stream));
-  double *bins = nullptr, *part = nullptr;
-  CF_HIP(hipMallocAsync((void **)&bins, nb * sizeof(double), st));
-  if (hipMallocAsync((void **)&part, ADJ_BLOCKS * sizeof(double), st) != hipSuccess) {
-    (void)hipFreeAsync(bins, st);
-    set_error("cf_loss_spectrum_adjoint: device allocation failed");
-    return CF_E_HIP;
-  }
+  static_assert(ADJ_BLOCKS <= 1025, "dL/dm partials live in the plan's reduction buffer");
+  double *bins = plan_bins(plan), *part = plan_red(plan);
   cf_status rc = cf_bin_power(plan, fk, bins, stream);
   std::vector<double> hb((size_t)nb), hw((size_t)nb), hp(ADJ_BLOCKS);
   if (rc == CF_OK && (hipMemcpyAsync(hb.data(), bins, nb * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -359,8 +356,6 @@ cf_status cf_loss_spectrum_adjoint(cf_spectrum_plan *plan, const cf_kg_params *p
   }
   if (rc == CF_OK && hipMemcpyAsync(hp.data(), part, ADJ_BLOCKS * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
     rc = CF_E_HIP;
-  (void)hipFreeAsync(bins, st);
-  (void)hipFreeAsync(part, st);
   hipError_t se = hipStreamSynchronize(st);
   if (rc != CF_OK) {
     set_error("cf_loss_spectrum_adjoint: device operation failed");

@@ -51,4 +51,11 @@ inline bool valid_N(int64_t N) { return N >= 4 && N % 2 == 0 && N <= 2048; }
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Scratch owned by a spectrum plan (cf_spectrum.hip), for library code that runs
+// with a plan: a reduction buffer of 1025 doubles and one spectrum of
+// cf_spectrum_bins(N) doubles. Calls that share a plan are ordered on one stream.
+double *plan_red(cf_spectrum_plan *p);
+double *plan_bins(cf_spectrum_plan *p);
+int64_t plan_size(const cf_spectrum_plan *p);  // its lattice size N
+
 }  // namespace cf

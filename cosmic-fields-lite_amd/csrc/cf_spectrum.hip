@@ -4,11 +4,12 @@
 // Binning without atomics. The half spectrum has N*N*(N/2+1) entries; entry
 // (a,b,c) belongs to bin s = a'^2 + b'^2 + c^2 with x' = min(x, N-x), and counts
 // twice unless c is 0 or N/2 (the full-cube loop of fdm3d.cpp:11-28 visits the
-// mirrored column too). At plan creation the host sorts the entry indices by s
-// (a stable counting sort, so each bin lists its entries in memory order). The
-// binning kernels give one wavefront to each bin: lane j sums entries j, j+64,
-// ... of the bin, then the wavefront reduces with a fixed butterfly. The result
-// does not depend on scheduling, so two runs give identical bits.
+// mirrored column too). A row (a, b) of the half spectrum has the row class
+// r = a'^2 + b'^2, so s = r + c^2. At plan creation the host sorts the N^2 rows by
+// r (4 B per row). Step A sums, for each class r, its whole rows (each read
+// contiguously, in sorted order) into Q[c][r]; step B forms P(s) as the sum over
+// ascending c of Q[c][s - c^2]. Every sum has a fixed order and no atomics are
+// used, so two runs give identical bits.
 #include <hipfft/hipfft.h>
 
 #include <cmath>
@@ -31,6 +32,7 @@ struct cf_spectrum_plan {
   int32_t *roff = nullptr;       // nr+1 offsets into rows
   int32_t *rows = nullptr;       // the N^2 rows (a*N + b) of the half spectrum sorted by r
   double *red = nullptr;         // reduction scratch (RED_BLOCKS + 1 doubles)
+  double *bins = nullptr;        // one spectrum (nb doubles) for callers inside the library
 };
 
 namespace cf {
@@ -329,7 +331,7 @@ cf_status cf_spectrum_plan_destroy(cf_spectrum_plan *p) {
   if (!p) return CF_OK;
   if (p->have_d2z) hipfftDestroy(p->d2z);
   if (p->have_z2d) hipfftDestroy(p->z2d);
-  void *bufs[] = {p->work, p->rbuf, p->cbuf0, p->cbuf1, p->roff, p->rows, p->red};
+  void *bufs[] = {p->work, p->rbuf, p->cbuf0, p->cbuf1, p->roff, p->rows, p->red, p->bins};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   delete p;
@@ -371,7 +373,8 @@ cf_status cf_spectrum_plan_create(int64_t N, cf_spectrum_plan **out) {
       hipMalloc((void **)&p->cbuf1, 2 * (size_t)p->nhalf * sizeof(double)) != hipSuccess ||
       hipMalloc((void **)&p->roff, ((size_t)p->nr + 1) * sizeof(int32_t)) != hipSuccess ||
       hipMalloc((void **)&p->rows, (size_t)(N * N) * sizeof(int32_t)) != hipSuccess ||
-      hipMalloc((void **)&p->red, (RED_BLOCKS + 1) * sizeof(double)) != hipSuccess) {
+      hipMalloc((void **)&p->red, (RED_BLOCKS + 1) * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&p->bins, (size_t)p->nb * sizeof(double)) != hipSuccess) {
     set_error("cf_spectrum_plan_create(N=%lld): device allocation failed", (long long)N);
     return fail(CF_E_NOMEM);
   }
@@ -544,18 +547,35 @@ cf_status cf_scaled_fourier_modes(cf_spectrum_plan *p, const double *f, const do
   return z2d_exec(p, p->cbuf0, out, st);
 }
 
+}  // extern "C"
+
+namespace cf {
+
+// x *= target / rms(x) with a caller-provided reduction scratch (RED_BLOCKS + 1 doubles).
+static cf_status rescale_rms_with(double *x, int64_t n, double target, double *red, hipStream_t st) {
+  hipLaunchKernelGGL(k_sumsq_partial, dim3(RED_BLOCKS), dim3(256), 0, st, x, n, red);
+  hipLaunchKernelGGL(k_rms_factor, dim3(1), dim3(256), 0, st, red, n, target);
+  hipLaunchKernelGGL(k_scale_by, dim3(grid_cap(n, 256)), dim3(256), 0, st, x, n, red + RED_BLOCKS);
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
+double *plan_red(cf_spectrum_plan *p) { return p->red; }
+double *plan_bins(cf_spectrum_plan *p) { return p->bins; }
+int64_t plan_size(const cf_spectrum_plan *p) { return p->N; }
+
+}  // namespace cf
+
+extern "C" {
+
 cf_status cf_rescale_rms(double *x, int64_t n, double target, cf_stream stream) {
   CF_REQUIRE(x && n > 0, "cf_rescale_rms: bad arguments");
-  // Uses a small private scratch; allocate per call (not a hot path).
+  // No plan to borrow from: a stream-ordered scratch of RED_BLOCKS + 1 doubles.
   double *red = nullptr;
   CF_HIP(hipMallocAsync((void **)&red, (RED_BLOCKS + 1) * sizeof(double), S(stream)));
-  hipLaunchKernelGGL(k_sumsq_partial, dim3(RED_BLOCKS), dim3(256), 0, S(stream), x, n, red);
-  hipLaunchKernelGGL(k_rms_factor, dim3(1), dim3(256), 0, S(stream), red, n, target);
-  hipLaunchKernelGGL(k_scale_by, dim3(grid_cap(n, 256)), dim3(256), 0, S(stream), x, n, red + RED_BLOCKS);
-  hipError_t le = hipGetLastError();
+  const cf_status rc = rescale_rms_with(x, n, target, red, S(stream));
   (void)hipFreeAsync(red, S(stream));
-  CF_HIP(le);
-  return CF_OK;
+  return rc;
 }
 
 cf_status cf_init_grf_state(cf_spectrum_plan *p, uint32_t seed, int kind, const double *params, double L,
@@ -591,7 +611,7 @@ cf_status cf_init_grf_state(cf_spectrum_plan *p, uint32_t seed, int kind, const 
     set_error("cf_init_grf_state: upload failed");
   }
   if (rc == CF_OK) rc = cf_grf(p, w, sqrtP, nullptr, state, stream);
-  if (rc == CF_OK && rms > 0.0) rc = cf_rescale_rms(state, N3, rms, stream);
+  if (rc == CF_OK && rms > 0.0) rc = rescale_rms_with(state, N3, rms, p->red, st);
   if (rc == CF_OK) rc = cf_fill_zero(w, (size_t)N3 * sizeof(double), stream);
   hipError_t se = hipStreamSynchronize(st);
   (void)hipFree(sqrtP);

@@ -7,7 +7,11 @@
  * Conventions
  *   - Lattice site (a,b,c) of an N^3 lattice is at N*N*a + N*b + c (c fastest),
  *     as IDX_OF in src/fdm3d.hpp:16. A state is [phi (N^3) | dphi/dt (N^3)].
- *   - All pointers named *_dev are device pointers owned by the caller.
+ *   - All pointers named *_dev are device pointers owned by the caller. The
+ *     library allocates device memory in the *_create calls (plans, ensembles,
+ *     variants, WKB), in cf_spectrum_plan_create for its own scratch, and, per
+ *     call, a stream-ordered scratch of at most 8 KB in cf_norm_inf and
+ *     cf_rescale_rms (hipMallocAsync / hipFreeAsync on the caller's stream).
  *   - Launches are asynchronous on the caller's stream (NULL = default stream);
  *     only cf_d2h, cf_sync and the *_host helpers block.
  *   - Every call returns CF_OK or an error code; cf_last_error() gives a
@@ -176,7 +180,11 @@ cf_status cf_ensemble_get_plan(const cf_ensemble *e, int64_t N, int *chunk, int 
  * (a step reads one buffer and writes the other, so the state is untouched), and
  * keeps the fastest plan. Blocking. */
 cf_status cf_ensemble_tune(cf_ensemble *e, double dt, int reps, int *grid_out);
-/* n_steps RK4 steps at t_k = t0 + (k0 + k) dt; use_graph: capture (or reuse) a hipGraph. */
+/* n_steps RK4 steps at t_k = t0 + (k0 + k) dt; use_graph: capture (or reuse) a hipGraph.
+ * Asynchronous: ordered after earlier work on `stream`, and later work on `stream`
+ * waits for it. Consecutive runs (segments of one evolution) need no host
+ * synchronisation between them; the graph is re-captured whenever the step table
+ * changes (another t0, k0, n_steps or dt, an eager run, a new plan). */
 cf_status cf_ensemble_run(cf_ensemble *e, double t0, int64_t k0, int64_t n_steps, double dt, int use_graph,
                           cf_stream stream);
 /* Energy-density power spectrum of realisation r at time t into bins_dev. */
