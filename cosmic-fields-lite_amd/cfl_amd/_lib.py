@@ -21,6 +21,8 @@ CF_E_UNSUPPORTED = 5
 CF_PREC_F64 = 0
 CF_PREC_F32C = 1
 
+CF_NOALLOC_MAX_N = 128  # largest N of cf_rk4_run_tb_f64_noalloc
+
 CF_ENS_GRID_FLAT = -1  # ensemble plan: one block per work unit
 
 CF_VAR_FIXED_CURVATURE = 1
@@ -86,6 +88,7 @@ _SIGS = [
     ("cf_rk4_run_f32c", st, [C.POINTER(KGParams), vp, vp, dbl, i64, i64, dbl, vp, vp]),
     ("cf_rk4_tb_work_bytes", C.c_size_t, [i64, C.c_int]),
     ("cf_rk4_run_tb_f64", st, [C.POINTER(KGParams), vp, dbl, i64, i64, dbl, C.c_int, vp, vp]),
+    ("cf_rk4_run_tb_f64_noalloc", st, [C.POINTER(KGParams), vp, dbl, i64, i64, dbl, C.c_int, vp]),
     ("cf_rk4_step_tb_f64", st, [C.POINTER(KGParams), vp, vp, dbl, dbl, C.c_int, vp]),
     ("cf_rk4_step_tb_f32c", st, [C.POINTER(KGParams), vp, vp, vp, vp, dbl, dbl, C.c_int, vp]),
     ("cf_rk4_run_tb_f32c", st, [C.POINTER(KGParams), vp, vp, dbl, i64, i64, dbl, C.c_int, vp, vp]),

@@ -9,6 +9,7 @@
 // state), so runs ping-pong between two state arrays.
 #include <cmath>
 
+#include <mutex>
 #include <vector>
 
 #include "cf_tb_body.hpp"
@@ -165,6 +166,32 @@ cf_status tb_step_variant(int var, const cf_kg_params *p, double f_a, const doub
   return CF_OK;
 }
 
+// Ping-pong buffer of the entry point that allocates nothing
+// (cf_rk4_run_tb_f64_noalloc): 2 N^3 doubles for N <= CF_NOALLOC_MAX_N, one copy per
+// GPU, held until the process ends. In host code `scratch` names a host-side
+// stand-in: kernels and copies take the device address from hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[2 * CF_NOALLOC_MAX_N * CF_NOALLOC_MAX_N * CF_NOALLOC_MAX_N];
+
+// scratch's device address on each device, looked up once per device with that
+// device current.
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64] = {};
+
+static cf_status scratch_on_current_device(double **out) {
+  int dev = 0;
+  CF_HIP(hipGetDevice(&dev));
+  CF_REQUIRE(dev >= 0 && dev < 64, "cf_rk4_run_tb_f64_noalloc: device ordinal %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    CF_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    CF_REQUIRE(ptr, "cf_rk4_run_tb_f64_noalloc: no device address for the scratch buffer");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = static_cast<double *>(g_scratch_ptr[dev]);
+  return CF_OK;
+}
+
 }  // namespace cf
 
 using namespace cf;
@@ -194,6 +221,29 @@ cf_status cf_rk4_run_tb_f64(const cf_kg_params *p, double *x, double t0, int64_t
     const size_t bytes = 2 * (size_t)p->N * p->N * p->N * sizeof(double);
     CF_HIP(hipMemcpyAsync(x, work, bytes, hipMemcpyDeviceToDevice, S(stream)));
   }
+  return CF_OK;
+}
+
+cf_status cf_rk4_run_tb_f64_noalloc(const cf_kg_params *p, double *x, double t0, int64_t k0, int64_t n_steps,
+                                    double dt, int chunk, cf_stream stream) {
+  CF_TRY(tb_check(p));
+  CF_REQUIRE(p->N <= CF_NOALLOC_MAX_N,
+             "cf_rk4_run_tb_f64_noalloc: N=%lld exceeds the preallocated buffer (N <= %d); use cf_rk4_run_tb_f64",
+             (long long)p->N, CF_NOALLOC_MAX_N);
+  CF_REQUIRE(x, "cf_rk4_run_tb_f64_noalloc: null state");
+  CF_REQUIRE(n_steps >= 0, "cf_rk4_run_tb_f64_noalloc: negative step count");
+  double *work = nullptr;
+  CF_TRY(scratch_on_current_device(&work));
+  const size_t bytes = 2 * (size_t)p->N * p->N * p->N * sizeof(double);
+  CF_REQUIRE(disjoint(x, work, bytes), "cf_rk4_run_tb_f64_noalloc: the state overlaps the library's buffer");
+  const TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N));
+  double *bufs[2] = {x, work};
+  for (int64_t s = 0; s < n_steps; ++s) {
+    const double t = t0 + static_cast<double>(k0 + s) * dt;
+    CF_TRY((tb_launch<double, false>(g, tb_step_host(p, t, dt), bufs[s & 1], nullptr, bufs[(s + 1) & 1], nullptr,
+                                      S(stream))));
+  }
+  if (n_steps & 1) CF_HIP(hipMemcpyAsync(x, work, bytes, hipMemcpyDeviceToDevice, S(stream)));
   return CF_OK;
 }
 

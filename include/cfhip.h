@@ -140,6 +140,18 @@ cf_status cf_rk4_run_tb_f64(const cf_kg_params *p, double *x_dev, double t0, int
                             double dt, int chunk, void *work_dev, cf_stream stream);
 cf_status cf_rk4_run_tb_f32c(const cf_kg_params *p, float *hi_dev, float *lo_dev, double t0, int64_t k0,
                              int64_t n_steps, double dt, int chunk, void *work_dev, cf_stream stream);
+/* cf_rk4_run_tb_f64 without a work buffer and without any allocation. No counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code
+ * under hipGraph capture, serving loops, harnesses that forbid hipMalloc). The
+ * ping-pong buffer is a static device array of the library (32 MiB, one per GPU,
+ * held until the process ends), so N <= CF_NOALLOC_MAX_N; larger N returns
+ * CF_E_INVALID. All calls on one GPU share that buffer: calls that overlap on one
+ * GPU (other streams, other host threads) race; calls ordered on one stream do not.
+ * The buffer needs no clearing (every step overwrites all of it). Same results as
+ * cf_rk4_run_tb_f64, bit for bit. */
+#define CF_NOALLOC_MAX_N 128
+cf_status cf_rk4_run_tb_f64_noalloc(const cf_kg_params *p, double *x_dev, double t0, int64_t k0, int64_t n_steps,
+                                    double dt, int chunk, cf_stream stream);
 /* One temporally blocked step out of place (x_out = step(x_in); the buffers must not
  * overlap; no work buffer): the form a stepper that swaps its state buffers uses. */
 cf_status cf_rk4_step_tb_f64(const cf_kg_params *p, const double *x_in_dev, double *x_out_dev, double t, double dt,
