@@ -166,6 +166,17 @@ __global__ void k_radial(const double *__restrict__ sb, int64_t nb, int64_t nr, 
   rad[j] = acc;
 }
 
+// Real field (a, b, c) at N^2 a + N b + c -> the FFTW padded layout of an in-place
+// real transform, rows of 2 (N/2 + 1) doubles (PADDED_IDX_OF, fdm3d.hpp:24). The
+// padding entries are left as they are.
+__global__ void k_pad_rows(const double *__restrict__ f, double *__restrict__ out, int64_t N, int64_t total) {
+  const int64_t W = 2 * (N / 2 + 1);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = i / N, c = i - row * N;
+    out[row * W + c] = f[i];
+  }
+}
+
 // Multiply the half spectrum by sqrt(P) per bin (random_field.cpp:60-73).
 __global__ void k_grf_scale(double2 *__restrict__ fk, const double *__restrict__ sqrtP, int64_t N,
                             int64_t H) {
@@ -420,6 +431,18 @@ cf_status cf_fft_z2d(cf_spectrum_plan *p, double *in, double *out, cf_stream str
   CF_TRY(plan_check(p));
   CF_REQUIRE(in && out, "cf_fft_z2d: null buffer");
   return z2d_exec(p, in, out, S(stream));
+}
+
+cf_status cf_fft_z2d_inplace(cf_spectrum_plan *p, double *inout, cf_stream stream) {
+  CF_TRY(plan_check(p));
+  CF_REQUIRE(inout, "cf_fft_z2d_inplace: null buffer");
+  // C2R into the plan's real scratch (the half spectrum in inout may be overwritten,
+  // as an in-place transform would), then spread the rows into the padded layout.
+  CF_TRY(z2d_exec(p, inout, p->rbuf, S(stream)));
+  const int64_t N = p->N, total = N * N * N;
+  hipLaunchKernelGGL(k_pad_rows, dim3(grid_cap(total, 256)), dim3(256), 0, S(stream), p->rbuf, inout, N, total);
+  CF_LAUNCHED();
+  return CF_OK;
 }
 
 // Step A into the plan's real scratch (free once the transform has run), step B into bins.

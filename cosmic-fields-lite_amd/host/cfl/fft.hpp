@@ -10,6 +10,7 @@
 #pragma once
 
 #include <cstdint>
+#include <stdexcept>
 
 #include "hip_vector.hpp"
 
@@ -51,6 +52,16 @@ class hipfftWrapper {
   void execute_z2d(hip_vector<double> &in, hip_vector<double> &out) {
     if (out.size() != (std::size_t)N * N * N) out.resize((std::size_t)N * N * N);
     check(cf_fft_z2d(plan_, in.data(), out.data(), default_stream()), "execute_z2d");
+  }
+  // In-place C2R (cuda_wrapper.cu:252-254): inout keeps its size 2 N^2 (N/2+1); the
+  // N^3 results sit in the FFTW padded layout, site (a,b,c) at padded_idx(a,b,c).
+  void execute_inplace_z2d(hip_vector<double> &inout) {
+    if (inout.size() != half_size()) throw std::runtime_error("execute_inplace_z2d: size is not 2 N^2 (N/2+1)");
+    check(cf_fft_z2d_inplace(plan_, inout.data(), default_stream()), "execute_inplace_z2d");
+  }
+  // PADDED_IDX_OF (fdm3d.hpp:24)
+  std::size_t padded_idx(long long a, long long b, long long c) const {
+    return (std::size_t)(2 * (N / 2 + 1)) * ((std::size_t)N * a + b) + c;
   }
 
  private:
