@@ -130,10 +130,42 @@ def cpu_baseline(N, steps):
     el = timed(1)
     cores = max(1, min(16, len(os.sched_getaffinity(0))))
     el_mt = timed(cores)
-    return {"value": N ** 3 * steps / el / 1e9, "unit": "GSite*RK4-step/s", "cores": 1, "kind": "port",
-            "sample": f"oracle FP64 RK4, KG-FRW N={N}^3, {steps} step(s), 1 thread, {el:.1f} s",
-            "all_cores": {"value": N ** 3 * steps / el_mt / 1e9, "cores": cores,
-                          "sample": f"same, a-loop over {cores} threads, {el_mt:.1f} s"}}
+    out = {"value": N ** 3 * steps / el / 1e9, "unit": "GSite*RK4-step/s", "cores": 1, "kind": "port",
+           "sample": f"oracle FP64 RK4, KG-FRW N={N}^3, {steps} step(s), 1 thread, {el:.1f} s",
+           "all_cores": {"value": N ** 3 * steps / el_mt / 1e9, "cores": cores,
+                         "sample": f"same, a-loop over {cores} threads, {el_mt:.1f} s"}}
+    ref = reference_integrator_timing(N, x, cores)
+    if ref:
+        out["reference_integrator"] = ref
+    return out
+
+
+def reference_integrator_timing(N, x, cores, steps=2):
+    """The reference's own integrator (vendored Boost.odeint integrate_const +
+    runge_kutta4_classic + src/odeint_eigen/eigen_operations.hpp, built from the
+    reference tree into oracle/_ref) around the oracle's right-hand side, timed inside
+    the binary (std::chrono::steady_clock around integrate_const), once under the
+    oracle's flags and once under the reference's (-O3 -ffast-math; x86-64-v3 for
+    -march=native). None when the binaries were not built."""
+    import subprocess
+    import tempfile
+    bins = {"reference_flags": os.path.join(ROOT, "oracle", "_ref", "odeint_ref_fastmath"),
+            "oracle_flags": os.path.join(ROOT, "oracle", "_ref", "odeint_ref")}
+    if not all(os.path.exists(b) for b in bins.values()):
+        return None
+    res = {"what": f"reference integrator around the oracle RHS, KG-FRW N={N}^3, {steps} RK4 steps per run "
+                   "(equations.cpp itself needs fftw3.h, absent here)", "unit": "GSite*RK4-step/s"}
+    with tempfile.TemporaryDirectory() as d:
+        x.tofile(os.path.join(d, "in.bin"))
+        for name, exe in bins.items():
+            res[name] = {}
+            for th in (1, cores):
+                r = subprocess.run([exe, os.path.join(d, "in.bin"), d, str(N), repr(float(N)), "1.0", "0.0", "1.0",
+                                    "0.5", "1.0", "1.0", repr(1.0 + steps * 0.05), "0.05", str(th)],
+                                   capture_output=True, text=True, check=True)
+                el = float(r.stderr.split("elapsed_s")[1].split()[0])
+                res[name][f"{th}_threads"] = N ** 3 * int(r.stdout) / el / 1e9
+    return res
 
 
 def ensemble_workload(args):

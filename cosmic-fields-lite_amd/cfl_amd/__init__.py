@@ -91,6 +91,13 @@ def norm_inf(v, stream=None):
     return out.value
 
 
+def count_nonfinite(v, stream=None):
+    """Number of NaN or infinite entries of a float64 device tensor (blocking)."""
+    k = C.c_int64(0)
+    check(lib.cf_count_nonfinite(_ptr(v), v.numel(), C.byref(k), _stream(stream)))
+    return k.value
+
+
 def energy_density_f64(N, x, rho, m, lam, inv, stream=None):
     check(lib.cf_energy_density_f64(N, _ptr(x), _ptr(rho), m, lam, inv, _stream(stream)))
 
@@ -209,6 +216,12 @@ class SpectrumPlan:
             out = torch.empty(self.N ** 3, dtype=torch.float64, device="cuda")
         check(lib.cf_fft_z2d(self.h, _ptr(fk), _ptr(out), _stream(stream)))
         return out
+
+    def z2d_inplace(self, fk, stream=None):
+        """In-place C2R: fk (2 N^2 (N/2+1) doubles) receives the unnormalised
+        transform in the FFTW padded layout (rows of 2 (N/2+1) doubles)."""
+        check(lib.cf_fft_z2d_inplace(self.h, _ptr(fk), _stream(stream)))
+        return fk
 
     def power_spectrum(self, f, out=None, stream=None, consume=False):
         """Bins of |f_k|^2. consume=True hands f to the transform without the copy

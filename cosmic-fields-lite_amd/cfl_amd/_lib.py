@@ -81,6 +81,7 @@ _SIGS = [
     ("cf_scale_sum2", st, [vp, vp, vp, dbl, dbl, i64, vp]),
     ("cf_scale_sum5", st, [vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, dbl, i64, vp]),
     ("cf_norm_inf", st, [vp, i64, C.POINTER(dbl), vp]),
+    ("cf_count_nonfinite", st, [vp, i64, C.POINTER(i64), vp]),
     ("cf_rk4_work_bytes", C.c_size_t, [i64, C.c_int]),
     ("cf_rk4_step_f64", st, [C.POINTER(KGParams), vp, dbl, dbl, vp, vp]),
     ("cf_rk4_run_f64", st, [C.POINTER(KGParams), vp, dbl, i64, i64, dbl, vp, vp]),
@@ -132,6 +133,7 @@ _SIGS = [
     ("cf_spectrum_bins", i64, [i64]),
     ("cf_fft_d2z", st, [vp, vp, vp, vp]),
     ("cf_fft_z2d", st, [vp, vp, vp, vp]),
+    ("cf_fft_z2d_inplace", st, [vp, vp, vp]),
     ("cf_power_spectrum", st, [vp, vp, vp, vp]),
     ("cf_power_spectrum_consume", st, [vp, vp, vp, vp]),
     ("cf_mode_power_spectrum", st, [vp, vp, dbl, dbl, dbl, vp, vp]),

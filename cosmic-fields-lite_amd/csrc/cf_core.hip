@@ -64,6 +64,23 @@ __global__ void k_absmax_partial(const double *__restrict__ v, int64_t n, double
   if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
+// Number of NaN and infinite values (exponent field all ones, independent of any
+// floating-point mode), one partial count per block.
+__global__ void k_nonfinite_partial(const double *__restrict__ v, int64_t n, unsigned long long *__restrict__ part) {
+  __shared__ unsigned long long sm[256];
+  unsigned long long k = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    k += ((__double_as_longlong(v[i]) >> 52) & 0x7ff) == 0x7ff;
+  sm[threadIdx.x] = k;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
 static int grid_for(int64_t n, int block, int max_blocks = 65536) {
   int64_t g = cdiv(n, block);
   if (g > max_blocks) g = max_blocks;
@@ -241,6 +258,28 @@ cf_status cf_norm_inf(const double *v, int64_t n, double *out, cf_stream stream)
   double m = 0.0;
   for (double x : h) m = x > m ? x : m;
   *out = m;
+  return CF_OK;
+}
+
+cf_status cf_count_nonfinite(const double *v, int64_t n, int64_t *count, cf_stream stream) {
+  CF_REQUIRE(v && count && n >= 0, "cf_count_nonfinite: bad arguments");
+  *count = 0;
+  if (n == 0) return CF_OK;
+  const int nb = grid_for(n, 256, 1024);
+  unsigned long long *part = nullptr;
+  CF_HIP(hipMallocAsync((void **)&part, sizeof(unsigned long long) * nb, S(stream)));
+  hipLaunchKernelGGL(k_nonfinite_partial, dim3(nb), dim3(256), 0, S(stream), v, n, part);
+  hipError_t le = hipGetLastError();
+  std::vector<unsigned long long> h(nb);
+  hipError_t ce = hipMemcpyAsync(h.data(), part, sizeof(unsigned long long) * nb, hipMemcpyDeviceToHost, S(stream));
+  hipError_t se = hipStreamSynchronize(S(stream));
+  (void)hipFreeAsync(part, S(stream));
+  CF_HIP(le);
+  CF_HIP(ce);
+  CF_HIP(se);
+  unsigned long long k = 0;
+  for (unsigned long long x : h) k += x;
+  *count = (int64_t)k;
   return CF_OK;
 }
 

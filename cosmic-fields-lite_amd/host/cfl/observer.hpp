@@ -10,10 +10,16 @@
 //
 // ConstStepObserver saves every K-th step (step-count based, so CPU and GPU
 // runs save at identical steps regardless of time rounding).
+//
+// Both observers have an opt-in finite check (check_finite = true): at every save
+// the state is scanned on the device and a NaN or infinity raises
+// cfl::NonFiniteState, which ends integrate_const. The reference has no such check
+// (it builds with -ffast-math).
 #pragma once
 
 #include <cstdio>
 #include <fstream>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -45,6 +51,23 @@ inline std::vector<double> load_vector_from_file(const std::string &filename) {
   return v;
 }
 
+// Raised by the observers' finite check.
+struct NonFiniteState : std::runtime_error {
+  double t;
+  long long count;
+  NonFiniteState(double t_, long long count_)
+      : std::runtime_error("state holds " + std::to_string(count_) + " NaN or infinite values at t = " +
+                           std::to_string(t_)),
+        t(t_), count(count_) {}
+};
+
+// Throws NonFiniteState if v holds a NaN or an infinity.
+inline void require_finite(const hip_vector<double> &v, double t) {
+  int64_t k = 0;
+  check(cf_count_nonfinite(v.data(), (int64_t)v.size(), &k, default_stream()), "cf_count_nonfinite");
+  if (k) throw NonFiniteState(t, (long long)k);
+}
+
 inline std::vector<double> to_host(const hip_vector<double> &d) {
   std::vector<double> h(d.size());
   copy_vector(h, d);
@@ -63,6 +86,7 @@ struct ConstIntervalObserver {
   int idx;
   std::string dir;
   double t_start, t_end, t_interval, t_last;
+  bool check_finite = false;  // scan the state for NaN / inf at every save
 
   template <typename Param>
   ConstIntervalObserver(const std::string &dir_, const Param &param, Equation &eqn)
@@ -79,6 +103,7 @@ struct ConstIntervalObserver {
   }
 
   void save(double t) {
+    if (check_finite) require_finite(workspace.state, t);
     const long long N = workspace.N;
     const double L = workspace.L, m = workspace.m;
     const double a_t = workspace.cosmology.a(t);

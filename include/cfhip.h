@@ -10,8 +10,9 @@
  *   - All pointers named *_dev are device pointers owned by the caller. The
  *     library allocates device memory in the *_create calls (plans, ensembles,
  *     variants, WKB), in cf_spectrum_plan_create for its own scratch, and, per
- *     call, a stream-ordered scratch of at most 8 KB in cf_norm_inf and
- *     cf_rescale_rms (hipMallocAsync / hipFreeAsync on the caller's stream).
+ *     call, a stream-ordered scratch of at most 8 KB in cf_norm_inf,
+ *     cf_count_nonfinite and cf_rescale_rms (hipMallocAsync / hipFreeAsync on the
+ *     caller's stream).
  *   - Launches are asynchronous on the caller's stream (NULL = default stream);
  *     only cf_d2h, cf_sync and the *_host helpers block.
  *   - Every call returns CF_OK or an error code; cf_last_error() gives a
@@ -103,6 +104,10 @@ cf_status cf_scale_sum5(double *v0_dev, const double *v1_dev, const double *v2_d
                         double a2, double a3, double a4, double a5, int64_t n, cf_stream stream);
 /* vector_space_norm_inf (src/odeint_thrust/thrust_algebra.hpp:51-64): max |v|, blocking. */
 cf_status cf_norm_inf(const double *v_dev, int64_t n, double *out_host, cf_stream stream);
+/* Number of NaN or infinite entries of v, blocking. No reference counterpart (the
+ * reference builds with -ffast-math and checks nothing): the observer's optional
+ * finite check. */
+cf_status cf_count_nonfinite(const double *v_dev, int64_t n, int64_t *count_host, cf_stream stream);
 
 /* ---------------------------------------------------------------- fused RK4 steps
  * One classic RK4 step (runge_kutta4_classic::do_step_impl,
@@ -321,6 +326,12 @@ int64_t cf_spectrum_bins(int64_t N); /* 3*(N/2)^2 + 1 */
  * z2d: out = unnormalised C2R of in; in is destroyed (as FFTW / cuFFT). */
 cf_status cf_fft_d2z(cf_spectrum_plan *plan, const double *in_dev, double *out_dev, cf_stream stream);
 cf_status cf_fft_z2d(cf_spectrum_plan *plan, double *in_dev, double *out_dev, cf_stream stream);
+/* execute_inplace_z2d (cuda_wrapper.cu:252-254; fftw_wrapper.hpp:64, fftw_wrapper.cpp:63-68):
+ * inout_dev holds the half spectrum (2 N^2 (N/2+1) doubles) and receives its
+ * unnormalised C2R in the FFTW padded layout: site (a, b, c) at
+ * 2(N/2+1)(N a + b) + c (PADDED_IDX_OF, fdm3d.hpp:24); the two padding doubles of
+ * each row are unspecified. */
+cf_status cf_fft_z2d_inplace(cf_spectrum_plan *plan, double *inout_dev, cf_stream stream);
 
 /* compute_power_spectrum (src/fdm3d.cpp:7-34; fdm3d_cuda.cu:161-176):
  * bins_dev[s] = sum over the full k-cube with |n|^2 = s of |f_k|^2. Deterministic. */

@@ -8,8 +8,9 @@
 // The right-hand side is the oracle's orc_rhs (the reference's equations.cpp
 // cannot be compiled here: it includes fftw3.h, which this image lacks).
 //
-// Usage: odeint_ref <in_state.bin> <out_dir> N L m lambda a1 H1 t1 t0 t_end dt
-//   in_state.bin: 2N^3 little-endian doubles. Writes out_dir/state.bin (final
+// Usage: odeint_ref <in_state.bin> <out_dir> N L m lambda a1 H1 t1 t0 t_end dt [threads]
+//   in_state.bin: 2N^3 little-endian doubles. threads (default 1): threads of the
+//   right-hand side's a-loop (the integrator's scale_sum passes stay on one). Writes out_dir/state.bin (final
 //   state), out_dir/times.bin (every time the observer was called) and
 //   out_dir/cosmo.bin (a, H, eta at 16 sample times t0 + k*(t_end-t0)/15).
 //   Prints the number of steps on stdout and "elapsed_s <seconds>" of the
@@ -30,6 +31,7 @@
 
 extern "C" void orc_rhs(int64_t N, const double *x, double *dxdt, double m, double lambda, double H,
                         double inv_ah2);
+extern "C" void orc_set_threads(int n);
 
 struct System {
   int64_t N;
@@ -58,10 +60,11 @@ static bool write_all(const std::string &path, const double *p, size_t n) {
 
 int main(int argc, char **argv) {
   using namespace boost::numeric::odeint;
-  if (argc != 13) {
-    fprintf(stderr, "usage: %s in_state.bin out_dir N L m lambda a1 H1 t1 t0 t_end dt\n", argv[0]);
+  if (argc != 13 && argc != 14) {
+    fprintf(stderr, "usage: %s in_state.bin out_dir N L m lambda a1 H1 t1 t0 t_end dt [threads]\n", argv[0]);
     return 2;
   }
+  orc_set_threads(argc == 14 ? atoi(argv[13]) : 1);
   const std::string in = argv[1], out = argv[2];
   const int64_t N = atoll(argv[3]);
   const double L = atof(argv[4]), m = atof(argv[5]), lambda = atof(argv[6]);

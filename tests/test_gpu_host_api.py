@@ -217,3 +217,43 @@ def test_config_frw_n256_2000_steps(cfl):
         assert torch.equal(xt, xc), f"temporally blocked vs odeint flow differ after {(block + 1) * 100} steps"
         print(f"step {(block + 1) * 100}: identical", flush=True)
     assert torch.isfinite(xc).all()
+
+
+REF_STYLE = os.path.join(ROOT, "cosmic-fields-lite_amd", "bin", "reference_style_equation")
+
+
+@pytest.mark.parametrize("N,L", [(32, 40.0), (48, 38.4)])
+def test_reference_style_comoving_equation(N, L):
+    """An equation written the reference's way against workspace.fft_wrapper
+    (execute_inplace_z2d + PADDED_IDX_OF, equations_cuda.cu:678-731) gives the
+    library's comoving-curvature dx/dt to 1e-12 of its maximum."""
+    print(f"case reference_style_equation N={N}", flush=True)
+    r = subprocess.run([REF_STYLE, str(N), repr(L), "1.0", "1.0", "0.05", "0.02"], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+FINITE = os.path.join(ROOT, "cosmic-fields-lite_amd", "bin", "observer_finite_check")
+
+
+def test_observer_finite_check(tmp_path):
+    """The observer's opt-in finite check: a clean state saves; planted inf / NaN /
+    -inf values raise cfl::NonFiniteState with their count, also out of
+    integrate_const."""
+    print("case observer_finite_check", flush=True)
+    r = subprocess.run([FINITE, str(tmp_path) + "/"], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("ok ") == 5
+
+
+def test_count_nonfinite(cfl):
+    import torch
+    x = torch.zeros(1 << 20, dtype=torch.float64, device="cuda")
+    assert cfl.count_nonfinite(x) == 0
+    x[0] = float("nan")
+    x[-1] = float("inf")
+    x[12345] = -float("inf")
+    x[7] = 1e308
+    assert cfl.count_nonfinite(x) == 3
