@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "cf_tb_body.hpp"
+#include "cf_tb_ring.hpp"
 #include "cf_tb_run.hpp"
 
 namespace cf {
@@ -48,6 +49,29 @@ __global__ __launch_bounds__(1024) void k_rk4_tb(TbGeom g, TbStep k, const T *__
   // Lane shifts for the c neighbours in FP64; LDS in the compensated mode, whose
   // float plane buffers are cheap and whose VALU load is the larger (measured).
   tb_unit<T, F32C, LAM, true, !F32C>(lds, u, k, xin, lin, xout, lout);
+}
+
+// FP64: the same step with the state streamed into an LDS ring TB_RING_DEPTH planes
+// ahead (cf_tb_ring.hpp; same bits as the register-prefetch body).
+constexpr int TB_RING_DEPTH = 4;
+
+template <bool LAM>
+__global__ __launch_bounds__(1024) void k_rk4_tb_ring(TbGeom g, TbStep k, const double *__restrict__ xin,
+                                                      double *__restrict__ xout) {
+  __shared__ __attribute__((aligned(16))) char smem[TbRing<TB_RING_DEPTH>::BYTES];
+  const int bid = tb_xcd_order(blockIdx.x, g.total);
+  const int tc = bid % g.nt;
+  const int rest = bid / g.nt;
+  const int tb = rest % g.nt;
+  const int ac = rest / g.nt;
+  TbUnit u;
+  u.N = g.N;
+  u.T = g.T;
+  u.b0 = (int64_t)tb * g.T;
+  u.c0 = (int64_t)tc * g.T;
+  u.a0 = (int64_t)ac * g.A;
+  u.a1 = (u.a0 + g.A < g.N) ? u.a0 + g.A : g.N;
+  tb_unit_ring<LAM, TB_RING_DEPTH>(smem, u, k, xin, xout);
 }
 
 // The same step for the fixed-curvature (Psi, N^3) and sqrt-potential equation
@@ -117,6 +141,14 @@ static int tb_default_chunk(int64_t N) {
 template <class T, bool F32C>
 static cf_status tb_launch(const TbGeom &g, const TbStep &k, const T *xin, const float *lin, T *xout,
                            float *lout, hipStream_t st) {
+  if constexpr (!F32C) {
+    if (k.lam != 0.0)
+      hipLaunchKernelGGL(k_rk4_tb_ring<true>, dim3(g.total), dim3(1024), 0, st, g, k, xin, xout);
+    else
+      hipLaunchKernelGGL(k_rk4_tb_ring<false>, dim3(g.total), dim3(1024), 0, st, g, k, xin, xout);
+    CF_LAUNCHED();
+    return CF_OK;
+  }
   if (k.lam != 0.0)
     hipLaunchKernelGGL((k_rk4_tb<T, F32C, true>), dim3(g.total), dim3(1024), 0, st, g, k, xin, lin, xout, lout);
   else
