@@ -47,9 +47,15 @@ BYTES_PER_SITE_STEP = {"f64": 272, "f32c": 152}
 DESIGN_BYTES_PER_SITE_STEP = {"f64": 256, "f32c": 152, "tb_f64": 32, "tb_f32c": 32}
 KERNEL_NAMES = {"f64": "cf::k_march<double, 1, S, false> (4 stage kernels)",
                 "f32c": "cf::k_march<float, 2, S, false> (4 stage kernels)",
-                "tb_f64": "cf::k_rk4_tb<double, false, false>",
+                "tb_f64": "cf::k_rk4_tb_ring<false>",
                 "tb_f32c": "cf::k_rk4_tb<float, true, false>"}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "rk4_tb_N512_pmc_traffic.md")
+# What bounds each path's kernel at N=512, from the committed PMC summaries: the FP32-comp
+# blocked kernel issues VALU 76% of its cycles (float <-> double conversions on top of the
+# FP64 arithmetic); the FP64 blocked kernel runs as fast with its arithmetic cut to a
+# skeleton, so its time is the memory traffic including the halo re-reads.
+LIMITER = {"tb_f32c": "vector-ALU issue", "tb_f64": "memory: HBM + L2 misses on halo re-reads",
+           "f32c": "hbm", "f64": "hbm"}
 
 
 def traffic_per_launch(kernel, N):
@@ -576,7 +582,7 @@ def main():
                      else None,
                      "canonical_bytes_per_site_step": canon,
                      "canonical_frac": canon * head["value"] / HBM_PEAK_GBS,
-                     "limiter": "vector-ALU issue" if head_mode.startswith("tb_") else "hbm",
+                     "limiter": LIMITER.get(head_mode, "hbm"),
                      "valu_busy": valu,
                      "note": "achieved = the bytes the kernel must move per site-step (read and write the "
                              "state once) x sites / average launch time; traffic = rocprofv3 FETCH_SIZE + "
