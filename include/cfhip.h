@@ -221,7 +221,7 @@ cf_status cf_loss_spectrum_adjoint(cf_spectrum_plan *plan, const cf_kg_params *p
                                    double *dldm, cf_stream stream);
 /* One step back: lam_dev holds dL/dx_{n+1} on entry, dL/dx_n on return, for the
  * step taken from x_dev at time t; dL/dm contributions go to part_dev (2048
- * doubles, summed by the caller in order). work_dev: 3 * 2N^3 doubles. */
+ * doubles, summed by the caller in order). work_dev: 2N^3 doubles. */
 cf_status cf_rk4_step_adjoint_f64(const cf_kg_params *p, const double *x_dev, double t, double dt, double *lam_dev,
                                   double *part_dev, void *work_dev, cf_stream stream);
 /* Forward n_steps from x0 (checkpoint every ckpt steps), loss at t0 + n_steps dt,

@@ -82,7 +82,7 @@ def test_sizes(lib, O):
     v = 2 * 128 ** 3 * 8
     for n_steps, ckpt in ((1024, 32), (23, 5), (0, 4)):
         nck = -(-n_steps // ckpt)
-        assert L.cf_adjoint_work_bytes(128, n_steps, ckpt) == (nck + ckpt + 7) * v + (4 * n_steps + 1) * 512 * 8
+        assert L.cf_adjoint_work_bytes(128, n_steps, ckpt) == (nck + ckpt + 5) * v + (4 * n_steps + 1) * 512 * 8
     assert L.cf_adjoint_work_bytes(128, 10, 0) == 0 and L.cf_adjoint_work_bytes(7, 10, 2) == 0
 
 

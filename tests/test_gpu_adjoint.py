@@ -95,12 +95,15 @@ def test_step_adjoint_vs_torch(cfl, O, lam):
     assert abs(host(part).sum() - gm.item()) <= 1e-12 * abs(gm.item())
 
 
-@pytest.mark.parametrize("N,lam", [(70, 0.25), (34, 0.0)])
+@pytest.mark.parametrize("N,lam", [(70, 0.25), (34, 0.0), (4, 0.1), (128, 0.0)])
 def test_step_adjoint_partial_tiles_vs_torch(cfl, O, N, lam):
-    """One step back at N = 70 (a full and a 6-column c tile, 18 b tiles of 4 with a
-    2-row last one) and N = 34 (one c tile with 30 idle lanes): the fused backward
-    kernels skip the columns outside the lattice and wrap the neighbours; the work
-    buffer is exactly the documented 3 x 2N^3 doubles."""
+    """One step back where the 28 x 28 output tiles of the backward kernel do not
+    divide the lattice: N = 70 (tiles of 28, 28, 14 per side), N = 34 (28, 6), N = 4
+    (one tile; the 32 x 32 region wraps the lattice eight times per side) and N = 128
+    (28 x 4 + 16, ten chunks along a). The kernel skips the columns outside the
+    lattice and wraps the neighbours; the work buffer is exactly the documented 2N^3
+    doubles, and every one of the 2048 dL/dm partial slots is written (they start as
+    NaN)."""
     import ctypes as C
     print(f"case step adjoint N={N} lam={lam}", flush=True)
     L, m, t, dt = float(N), 1.2, 1.4, 0.07
@@ -110,8 +113,8 @@ def test_step_adjoint_partial_tiles_vs_torch(cfl, O, N, lam):
     lamv = rng.standard_normal(2 * N ** 3)
     p = cfl.kg_params(N, L, m, lam, cfl.radiation(**FRW))
     ld = torch.from_numpy(lamv.copy()).cuda()
-    part = torch.zeros(4 * 512, dtype=torch.float64, device="cuda")
-    work = torch.empty(3 * 2 * N ** 3, dtype=torch.float64, device="cuda")
+    part = torch.full((4 * 512,), float("nan"), dtype=torch.float64, device="cuda")
+    work = torch.empty(2 * N ** 3, dtype=torch.float64, device="cuda")
     cfl.check(cfl.lib.cf_rk4_step_adjoint_f64(C.byref(p), cfl._ptr(torch.from_numpy(x).cuda()), t, dt, cfl._ptr(ld),
                                               cfl._ptr(part), cfl._ptr(work), cfl._stream()))
     xt = torch.tensor(x, requires_grad=True)
@@ -122,7 +125,7 @@ def test_step_adjoint_partial_tiles_vs_torch(cfl, O, N, lam):
     assert abs(host(part).sum() - gm.item()) <= 1e-12 * abs(gm.item())
 
 
-@pytest.mark.parametrize("n_steps,ckpt,lam", [(24, 8, 0.0), (23, 5, 0.2)])
+@pytest.mark.parametrize("n_steps,ckpt,lam", [(24, 8, 0.0), (23, 5, 0.2), (7, 16, 0.0), (1, 1, 0.2)])
 def test_full_chain_vs_torch(cfl, O, n_steps, ckpt, lam):
     """Checkpointed forward + backward through n_steps equals torch autograd of the
     same RK4 chain + loss (float64, CPU)."""
