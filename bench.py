@@ -178,8 +178,9 @@ def ensemble_workload(args):
     """96 seeded realisations (seeds 100-131: N=64, 132-163: N=96, 164-195: N=128,
     L=N), FRW Klein-Gordon with m=1 plus lambda phi^4/4 (lambda=0.1), FP32-comp,
     P(k) ~ exp(-k^2/(2*0.5^2)), rms 0.5, RK4 dt=0.05, advanced by one persistent
-    kernel per step captured in a hipGraph. Reports aggregate GSite*step/s and the
-    ensemble-averaged density spectrum of each N group."""
+    kernel per step captured in a hipGraph, with the plan (grid, and per N the block
+    region shape and a-chunk) picked by the autotuner. Reports aggregate
+    GSite*step/s and the ensemble-averaged density spectrum of each N group."""
     import numpy as np
     import torch
     import cfl_amd as cfl
@@ -191,6 +192,7 @@ def ensemble_workload(args):
     ens.init_grf(seeds, cfl.CF_SPEC_GAUSS, (0.5,), 0.5)
     tuned_grid = ens.tune(dt, reps=3)
     plan = {n: ens.get_plan(n)[0] for n in (64, 96, 128)}
+    regions = {n: "24x40" if ens.get_region(n) == cfl.CF_ENS_REGION_24X40 else "32x32" for n in (64, 96, 128)}
     # capture + first launch outside the timed region (the graph is reused)
     ens.run(t0, 0, steps, dt, use_graph=True)
     torch.cuda.synchronize()
@@ -226,7 +228,7 @@ def ensemble_workload(args):
         "ms_per_step": ms / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "storage": "f32 hi + f32 compensation", "data": "synthetic (Gaussian random fields, seeds 100-195)",
         "config": {"workload": "ensemble 32xN=64 + 32xN=96 + 32xN=128, L=N, FRW KG m=1 lambda=0.1, dt=0.05",
-                   "sites": ens.sites, "tuned_grid": "flat (one block per unit)" if tuned_grid == cfl.CF_ENS_GRID_FLAT else tuned_grid, "tuned_chunks": plan, "graph_wall_s": wall},
+                   "sites": ens.sites, "tuned_grid": "flat (one block per unit)" if tuned_grid == cfl.CF_ENS_GRID_FLAT else tuned_grid, "tuned_chunks": plan, "tuned_regions": regions, "graph_wall_s": wall},
         "ensemble_avg_density_spectra": summary,
     }
     print(json.dumps(line), flush=True)

@@ -12,7 +12,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import (CF_ENS_GRID_FLAT, CF_PREC_F32C, CF_PREC_F64, CF_SPEC_DERIV_POWER_CUTOFF, CF_SPEC_GAUSS,  # noqa: F401
+from ._lib import (CF_ENS_GRID_FLAT, CF_ENS_REGION_24X40, CF_ENS_REGION_32X32, CF_PREC_F32C, CF_PREC_F64, CF_SPEC_DERIV_POWER_CUTOFF, CF_SPEC_GAUSS,  # noqa: F401
                    CF_SPEC_KM3_GAUSS, CF_SPEC_POWER_CUTOFF, CF_SPEC_SCALE_INV,
                    CF_VAR_APPROX_COMOVING_CURVATURE, CF_VAR_COMOVING_CURVATURE, CF_VAR_FIXED_CURVATURE,
                    CF_VAR_SQRT_POTENTIAL, CfError, Cosmology, KGParams, check)
@@ -353,6 +353,14 @@ class Ensemble:
         c, g = C.c_int(), C.c_int()
         check(lib.cf_ensemble_get_plan(self.h, N, C.byref(c), C.byref(g)))
         return c.value, g.value
+
+    def set_region(self, N, region):
+        check(lib.cf_ensemble_set_region(self.h, N, region))
+
+    def get_region(self, N):
+        r = C.c_int()
+        check(lib.cf_ensemble_get_region(self.h, N, C.byref(r)))
+        return r.value
 
     def tune(self, dt, reps=3):
         g = C.c_int()

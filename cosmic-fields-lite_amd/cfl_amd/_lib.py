@@ -24,6 +24,8 @@ CF_PREC_F32C = 1
 CF_NOALLOC_MAX_N = 128  # largest N of cf_rk4_run_tb_f64_noalloc
 
 CF_ENS_GRID_FLAT = -1  # ensemble plan: one block per work unit
+CF_ENS_REGION_32X32 = 0  # ensemble block region (b rows x c columns): 24 x 24 tiles
+CF_ENS_REGION_24X40 = 1  # 16 x 32 tiles
 
 CF_VAR_FIXED_CURVATURE = 1
 CF_VAR_SQRT_POTENTIAL = 2
@@ -100,6 +102,8 @@ _SIGS = [
     ("cf_ensemble_init_grf", st, [vp, vp, C.c_int, vp, dbl, vp]),
     ("cf_ensemble_set_plan", st, [vp, i64, C.c_int, C.c_int]),
     ("cf_ensemble_get_plan", st, [vp, i64, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("cf_ensemble_set_region", st, [vp, i64, C.c_int]),
+    ("cf_ensemble_get_region", st, [vp, i64, C.POINTER(C.c_int)]),
     ("cf_ensemble_tune", st, [vp, dbl, C.c_int, C.POINTER(C.c_int)]),
     ("cf_ensemble_run", st, [vp, dbl, i64, i64, dbl, C.c_int, vp]),
     ("cf_ensemble_density_spectrum", st, [vp, C.c_int, dbl, vp, vp]),

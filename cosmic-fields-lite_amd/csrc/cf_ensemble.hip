@@ -36,9 +36,24 @@ struct EnsReal {
 
 struct EnsUnit {
   int32_t real;
-  int32_t T;
+  int32_t T, Tc;   // output tile edges along b and c
+  int32_t region;  // CF_ENS_REGION_*: the block's region shape
   int64_t b0, c0, a0, a1;
 };
+
+// Region shapes of a unit's block (rows along b x columns along c): 32 x 32 (24 x 24
+// tiles, 1024 threads) and 24 x 40 (16 x 32 tiles, 960 threads: 1.88 region columns
+// per output site against 1.78, but tiles that divide N = 64 and 128 where 24 x 24
+// leaves 27% of the covered area outside the lattice).
+constexpr int ENS_R1_B = 24, ENS_R1_C = 40;
+inline int ens_tile_b(int region, int64_t N) {
+  const int t = region == CF_ENS_REGION_24X40 ? ENS_R1_B - 2 * TBH : TBR - 2 * TBH;
+  return N < t ? (int)N : t;
+}
+inline int ens_tile_c(int region, int64_t N) {
+  const int t = region == CF_ENS_REGION_24X40 ? ENS_R1_C - 2 * TBH : TBR - 2 * TBH;
+  return N < t ? (int)N : t;
+}
 
 struct EnsStepCoef {
   double a[4], H[4];  // a(t) and H(t) at the four stage times
@@ -60,11 +75,10 @@ __device__ __forceinline__ V wave_uniform(const V &v) {
 }
 
 // One work unit of the step: its lattice's step scalars, then the temporally
-// blocked body.
-template <class T, bool F32C, bool LAM, bool DEFER, bool LANE_NBR = true>
-__device__ __forceinline__ void ens_unit(TbLds<T> &lds, const EnsReal *__restrict__ reals, const EnsUnit &unit,
+// blocked body over a region of RB x RC columns.
+template <class T, bool F32C, bool LAM, bool DEFER, bool LANE_NBR = true, int RB = TBR, int RC = TBR>
+__device__ __forceinline__ void ens_unit(TbLds<T> &lds, const EnsReal *__restrict__ reals, const EnsUnit &e,
                                          const EnsStepCoef &sc, int parity, double c_m, double lam, double dt) {
-  const EnsUnit e = wave_uniform(unit);
   const EnsReal &R = reals[e.real];
   // Select this step's buffers first, then make each pointer uniform: indexing
   // x[parity] after the struct is in registers leaves the compiler treating the
@@ -100,7 +114,20 @@ __device__ __forceinline__ void ens_unit(TbLds<T> &lds, const EnsReal *__restric
   u.a0 = e.a0;
   u.a1 = e.a1;
   u.T = e.T;
-  tb_unit<T, F32C, LAM, DEFER, LANE_NBR>(lds, u, k, xin, lin, xout, lout);
+  u.Tc = e.Tc;
+  tb_unit<T, F32C, LAM, DEFER, LANE_NBR, TB_KG, false, RB, RC>(lds, u, k, xin, lin, xout, lout);
+}
+
+// A unit in its region shape (a wave-uniform branch; lane neighbours only in rows of
+// 32 columns).
+template <class T, bool F32C, bool LAM, bool DEFER, bool LANE_NBR>
+__device__ __forceinline__ void ens_unit_any(TbLds<T> &lds, const EnsReal *__restrict__ reals, const EnsUnit &unit,
+                                             const EnsStepCoef &sc, int parity, double c_m, double lam, double dt) {
+  const EnsUnit e = wave_uniform(unit);
+  if (e.region == CF_ENS_REGION_24X40)
+    ens_unit<T, F32C, LAM, DEFER, false, ENS_R1_B, ENS_R1_C>(lds, reals, e, sc, parity, c_m, lam, dt);
+  else
+    ens_unit<T, F32C, LAM, DEFER, LANE_NBR>(lds, reals, e, sc, parity, c_m, lam, dt);
 }
 
 // Persistent form: a fixed grid; block b runs units unit_off[b] .. unit_off[b+1]-1
@@ -116,7 +143,7 @@ __global__ __launch_bounds__(1024) void k_ens_tb(const EnsReal *__restrict__ rea
   const EnsStepCoef sc = steps[step];
   const int beg = unit_off[blockIdx.x], end = unit_off[blockIdx.x + 1];
   for (int i = beg; i < end; ++i) {
-    ens_unit<T, F32C, LAM, false>(lds, reals, units[i], sc, parity, c_m, lam, dt);
+    ens_unit_any<T, F32C, LAM, false, true>(lds, reals, units[i], sc, parity, c_m, lam, dt);
     __syncthreads();  // the next unit rewrites the plane buffers
   }
 }
@@ -140,7 +167,7 @@ __global__ __launch_bounds__(1024) void k_ens_tb_flat(const EnsReal *__restrict_
     const int x = u & 7, i = u >> 3;
     u = ((i / ENS_XCD_GROUP) * 8 + x) * ENS_XCD_GROUP + (i % ENS_XCD_GROUP);
   }
-  ens_unit<T, F32C, LAM, true, !F32C>(lds, reals, units[u], steps[step], parity, c_m, lam, dt);
+  ens_unit_any<T, F32C, LAM, true, !F32C>(lds, reals, units[u], steps[step], parity, c_m, lam, dt);
 }
 
 }  // namespace cf
@@ -154,7 +181,8 @@ struct cf_ensemble {
   std::vector<EnsReal> reals;
   std::vector<void *> allocs;
   EnsReal *d_reals = nullptr;
-  std::map<int64_t, int> chunk;  // planes per unit, per lattice size
+  std::map<int64_t, int> chunk;   // planes per unit, per lattice size
+  std::map<int64_t, int> region;  // CF_ENS_REGION_*, per lattice size
   int grid = 0;
   int n_units = 0;
   int32_t *d_unit_off = nullptr;
@@ -203,21 +231,25 @@ static cf_status ens_build(cf_ensemble *e) {
   std::vector<int> cost;
   for (int r = 0; r < e->n; ++r) {
     const int64_t N = e->reals[r].N;
-    const int T = tb_tile(N);
-    const int64_t nt = cdiv(N, T);
+    const int reg = e->region[N];
+    const int Tb = ens_tile_b(reg, N), Tc = ens_tile_c(reg, N);
+    const int64_t ntb = cdiv(N, Tb), ntc = cdiv(N, Tc);
     const int A = e->chunk[N];
+    const int waves = reg == CF_ENS_REGION_24X40 ? (ENS_R1_B * ENS_R1_C + 63) / 64 : TBR * TBR / 64;
     for (int64_t a0 = 0; a0 < N; a0 += A)
-      for (int64_t tb = 0; tb < nt; ++tb)
-        for (int64_t tc = 0; tc < nt; ++tc) {
+      for (int64_t tb = 0; tb < ntb; ++tb)
+        for (int64_t tc = 0; tc < ntc; ++tc) {
           EnsUnit u;
           u.real = r;
-          u.T = T;
-          u.b0 = tb * T;
-          u.c0 = tc * T;
+          u.T = Tb;
+          u.Tc = Tc;
+          u.region = reg;
+          u.b0 = tb * Tb;
+          u.c0 = tc * Tc;
           u.a0 = a0;
           u.a1 = std::min<int64_t>(a0 + A, N);
           units.push_back(u);
-          cost.push_back((int)(u.a1 - u.a0) + 8);
+          cost.push_back(((int)(u.a1 - u.a0) + 8) * waves);
         }
   }
   std::vector<int> order(units.size());
@@ -441,6 +473,7 @@ cf_status cf_ensemble_create(int n, const int64_t *N, const double *L, double m,
     }
     e->reals.push_back(R);
     if (!e->chunk.count(N[r])) e->chunk[N[r]] = ens_default_chunk(N[r]);
+    if (!e->region.count(N[r])) e->region[N[r]] = CF_ENS_REGION_32X32;
   }
   if (hipMalloc((void **)&e->d_reals, n * sizeof(EnsReal)) != hipSuccess ||
       hipMemcpy(e->d_reals, e->reals.data(), n * sizeof(EnsReal), hipMemcpyHostToDevice) != hipSuccess ||
@@ -487,6 +520,23 @@ cf_status cf_ensemble_get_plan(const cf_ensemble *e, int64_t N, int *chunk, int 
   return CF_OK;
 }
 
+cf_status cf_ensemble_set_region(cf_ensemble *e, int64_t N, int region) {
+  CF_REQUIRE(e, "cf_ensemble_set_region: null ensemble");
+  CF_REQUIRE(e->region.count(N), "cf_ensemble_set_region: no lattice of size %lld", (long long)N);
+  CF_REQUIRE(region == CF_ENS_REGION_32X32 || region == CF_ENS_REGION_24X40,
+             "cf_ensemble_set_region: unknown region shape %d", region);
+  e->region[N] = region;
+  return ens_build(e);
+}
+
+cf_status cf_ensemble_get_region(const cf_ensemble *e, int64_t N, int *region) {
+  CF_REQUIRE(e && region, "cf_ensemble_get_region: bad arguments");
+  auto it = e->region.find(N);
+  CF_REQUIRE(it != e->region.end(), "cf_ensemble_get_region: no lattice of size %lld", (long long)N);
+  *region = it->second;
+  return CF_OK;
+}
+
 cf_status cf_ensemble_init_grf(cf_ensemble *e, const uint32_t *seeds, int kind, const double *params, double rms,
                                cf_stream stream) {
   CF_REQUIRE(e && seeds, "cf_ensemble_init_grf: bad arguments");
@@ -522,21 +572,29 @@ cf_status cf_ensemble_tune(cf_ensemble *e, double dt, int reps, int *grid_out) {
   }
   e->grid = best_g;
   CF_TRY(ens_build(e));
+  // per lattice size, every (region shape, chunk) pair, the other sizes' plans held
   for (auto &kv : e->chunk) {
     const int64_t N = kv.first;
-    int best_a = kv.second;
+    int best_a = kv.second, best_r = e->region[N];
     CF_TRY(ens_time(e, reps, dt, &best));
-    for (int A : {8, 16, 32, 64, 128, 256}) {
-      if (A > N || A == best_a) continue;
-      e->chunk[N] = A;
-      CF_TRY(ens_build(e));
-      CF_TRY(ens_time(e, reps, dt, &ms));
-      if (ms < best) {
-        best = ms;
-        best_a = A;
+    std::vector<int> chunks;
+    for (int A : {8, 16, 32, 64, 128, 256, (int)N})
+      if (A <= N && std::find(chunks.begin(), chunks.end(), A) == chunks.end()) chunks.push_back(A);
+    for (int reg : {CF_ENS_REGION_32X32, CF_ENS_REGION_24X40})
+      for (int A : chunks) {
+        if (A == best_a && reg == best_r) continue;
+        e->chunk[N] = A;
+        e->region[N] = reg;
+        CF_TRY(ens_build(e));
+        CF_TRY(ens_time(e, reps, dt, &ms));
+        if (ms < best) {
+          best = ms;
+          best_a = A;
+          best_r = reg;
+        }
       }
-    }
     e->chunk[N] = best_a;
+    e->region[N] = best_r;
     CF_TRY(ens_build(e));
   }
   if (grid_out) *grid_out = e->grid;

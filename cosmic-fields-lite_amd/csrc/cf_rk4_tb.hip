@@ -42,6 +42,7 @@ __global__ __launch_bounds__(1024) void k_rk4_tb(TbGeom g, TbStep k, const T *__
   TbUnit u;
   u.N = g.N;
   u.T = g.T;
+  u.Tc = g.T;
   u.b0 = (int64_t)tb * g.T;
   u.c0 = (int64_t)tc * g.T;
   u.a0 = (int64_t)ac * g.A;
@@ -67,6 +68,7 @@ __global__ __launch_bounds__(1024) void k_rk4_tb_ring(TbGeom g, TbStep k, const 
   TbUnit u;
   u.N = g.N;
   u.T = g.T;
+  u.Tc = g.T;
   u.b0 = (int64_t)tb * g.T;
   u.c0 = (int64_t)tc * g.T;
   u.a0 = (int64_t)ac * g.A;
@@ -89,6 +91,7 @@ __global__ __launch_bounds__(1024) void k_rk4_tb_var(TbGeom g, TbStep k, const d
   TbUnit u;
   u.N = g.N;
   u.T = g.T;
+  u.Tc = g.T;
   u.b0 = (int64_t)tb * g.T;
   u.c0 = (int64_t)tc * g.T;
   u.a0 = (int64_t)ac * g.A;

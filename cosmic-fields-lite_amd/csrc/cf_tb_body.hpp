@@ -4,7 +4,9 @@
 //
 // Geometry. A block is 32 x 32 threads over a 32 x 32 (b, c) region of the
 // lattice: an output tile of T x T sites (T = 24, or N for small lattices) plus
-// a halo of 4 sites on every side, one per stencil stage. The block marches
+// a halo of 4 sites on every side, one per stencil stage. Other region shapes
+// RB x RC (rows x columns, RB RC <= 1024) give tiles of (RB - 8) x (RC - 8); with
+// RB RC < 1024 the threads past the region idle through the unit. The block marches
 // along a through a chunk of planes [a0, a1). Each thread owns one (b, c)
 // column and keeps, in registers, the short history of planes every stage
 // needs along a:
@@ -64,11 +66,12 @@ struct TbPsi {
   int64_t G, sd, hs;
 };
 
-// One work unit: lattice size, output tile origin and edge, plane chunk.
+// One work unit: lattice size, output tile origin and edges (T along b, Tc along
+// c), plane chunk.
 struct TbUnit {
   int64_t N;
   int64_t b0, c0, a0, a1;
-  int T;
+  int T, Tc;
 };
 
 __device__ __forceinline__ double tb_rhs(double phi, double pa1, double pam, double pb1, double pbm, double pc1,
@@ -189,14 +192,23 @@ struct TbPending {
 template <class T>
 using TbLds = T[2][4][TBR][TBR];
 
-template <class T, bool F32C, bool LAM, bool DEFER = true, bool LANE_NBR = true, int VAR = TB_KG, bool RING = false>
-__device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const TbStep &k, const T *__restrict__ xin,
-                                        const float *__restrict__ lin, T *__restrict__ xout,
-                                        float *__restrict__ lout, const double *__restrict__ psi = nullptr,
-                                        const TbPsi cps = TbPsi{}) {
+template <class T, bool F32C, bool LAM, bool DEFER = true, bool LANE_NBR = true, int VAR = TB_KG, bool RING = false,
+          int RB = TBR, int RC = TBR>
+__device__ __forceinline__ void tb_unit(TbLds<T> &lds_all, const TbUnit &u, const TbStep &k,
+                                        const T *__restrict__ xin, const float *__restrict__ lin,
+                                        T *__restrict__ xout, float *__restrict__ lout,
+                                        const double *__restrict__ psi = nullptr, const TbPsi cps = TbPsi{}) {
   static_assert(VAR == TB_KG || !F32C, "the equation variants are FP64");
+  static_assert(RB * RC <= TBR * TBR && RB > 2 * TBH && RC > 2 * TBH, "region shape");
+  static_assert(!LANE_NBR || RC == TBR, "lane neighbours need rows of 32 lanes");
+  // the plane buffers of this shape, in the storage of the 32 x 32 ones
+  T(&lds)[2][4][RB][RC] = *reinterpret_cast<T(*)[2][4][RB][RC]>(&lds_all);
   const int tid = threadIdx.x;
-  const int r = tid >> 5, cc = tid & 31;
+  // threads past the region (RB RC < 1024) act as copies of its last row that
+  // write no plane buffer and store nothing
+  constexpr bool FULL = RB * RC == TBR * TBR;
+  const bool live = FULL || tid < RB * RC;
+  const int r = live ? tid / RC : RB - 1, cc = live ? tid % RC : RC - 1;
   static_assert(!F32C || sizeof(T) == sizeof(float), "the compensated mode stores float");
   // In the compensated mode every value carried from plane to plane is exactly
   // representable in float (the state hi, the rounded stage inputs and the rounded
@@ -210,18 +222,18 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
   int64_t c = (u.c0 - TBH + cc) % N;
   if (c < 0) c += N;
   const int64_t col = b * N + c;
-  const bool out_ok = r >= TBH && r < TBH + u.T && u.b0 + (r - TBH) < N && cc >= TBH && cc < TBH + u.T &&
-                      u.c0 + (cc - TBH) < N;
-  const int rm = r > 0 ? r - 1 : 0, rp = r < TBR - 1 ? r + 1 : TBR - 1;
-  const int cm = cc > 0 ? cc - 1 : 0, cp = cc < TBR - 1 ? cc + 1 : TBR - 1;
+  const bool out_ok = live && r >= TBH && r < TBH + u.T && u.b0 + (r - TBH) < N && cc >= TBH &&
+                      cc < TBH + u.Tc && u.c0 + (cc - TBH) < N;
+  const int rm = r > 0 ? r - 1 : 0, rp = r < RB - 1 ? r + 1 : RB - 1;
+  const int cm = cc > 0 ? cc - 1 : 0, cp = cc < RC - 1 ? cc + 1 : RC - 1;
   // c+1 / c-1 neighbour of a stage input: from the adjacent lanes (LANE_NBR) or
   // from the LDS plane buffer (the same values; which is faster depends on the
   // kernel's mix of LDS and VALU work)
-  auto cnext = [&](Hv v, const T(*pl)[TBR]) -> double {
+  auto cnext = [&](Hv v, const T(*pl)[RC]) -> double {
     if constexpr (LANE_NBR) return (double)tb_lane_next(v);
     else return (double)pl[r][cp];
   };
-  auto cprev = [&](Hv v, const T(*pl)[TBR]) -> double {
+  auto cprev = [&](Hv v, const T(*pl)[RC]) -> double {
     if constexpr (LANE_NBR) return (double)tb_lane_prev(v);
     else return (double)pl[r][cm];
   };
@@ -334,7 +346,7 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     // unrolled body lengthens live ranges past the register budget of 1024-thread
     // blocks)
     if constexpr (RING) __builtin_amdgcn_sched_barrier(0);
-    T(*buf)[TBR][TBR] = lds[S];
+    T(*buf)[RB][RC] = lds[S];
     TbPending<T> &q = Q[S];
     double lo_f = 0.0, lo_p = 0.0;
     if constexpr (F32C) {
@@ -353,10 +365,12 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds, const TbUnit &u, const Tb
     XP[IN] = (Hv)xin[N3 + off_ld + col];
     if constexpr (VAR == TB_FIXED_CURVATURE) PS[IN] = psi[off_ld + col];
 
-    buf[0][r][cc] = (T)XF[I1];
-    buf[1][r][cc] = (T)Y1F[I1];
-    buf[2][r][cc] = (T)Y2F[I1];
-    buf[3][r][cc] = (T)Y3F[I1];
+    if (live) {
+      buf[0][r][cc] = (T)XF[I1];
+      buf[1][r][cc] = (T)Y1F[I1];
+      buf[2][r][cc] = (T)Y2F[I1];
+      buf[3][r][cc] = (T)Y3F[I1];
+    }
     __syncthreads();
 
     // the stages' Psi: fixed curvature from its plane history, comoving from the

@@ -66,7 +66,7 @@ __device__ __forceinline__ void tb_unit_ring(char *smem, const TbUnit &u, const 
   int64_t c = (u.c0 - TBH + cc) % N;
   if (c < 0) c += N;
   const int64_t col = b * N + c;
-  const bool out_ok = r >= TBH && r < TBH + u.T && u.b0 + (r - TBH) < N && cc >= TBH && cc < TBH + u.T &&
+  const bool out_ok = r >= TBH && r < TBH + u.T && u.b0 + (r - TBH) < N && cc >= TBH && cc < TBH + u.Tc &&
                       u.c0 + (cc - TBH) < N;
   const int rm = r > 0 ? r - 1 : 0, rp = r < TBR - 1 ? r + 1 : TBR - 1;
 

@@ -193,9 +193,17 @@ cf_status cf_ensemble_init_grf(cf_ensemble *e, const uint32_t *seeds, int kind, 
 #define CF_ENS_GRID_FLAT (-1)
 cf_status cf_ensemble_set_plan(cf_ensemble *e, int64_t N, int chunk, int grid);
 cf_status cf_ensemble_get_plan(const cf_ensemble *e, int64_t N, int *chunk, int *grid);
-/* Autotuner: times candidate grids (persistent and flat), then candidate chunks per N, on the device
- * (a step reads one buffer and writes the other, so the state is untouched), and
- * keeps the fastest plan. Blocking. */
+/* Region shape of the blocks that advance lattices of size N (rows along b x columns
+ * along c): CF_ENS_REGION_32X32 (24 x 24 output tiles, the default) or
+ * CF_ENS_REGION_24X40 (16 x 32 tiles, which divide N = 64 and 128; 24 x 24 tiles
+ * leave 27% of the covered area outside such lattices). Same bits either way. */
+#define CF_ENS_REGION_32X32 0
+#define CF_ENS_REGION_24X40 1
+cf_status cf_ensemble_set_region(cf_ensemble *e, int64_t N, int region);
+cf_status cf_ensemble_get_region(const cf_ensemble *e, int64_t N, int *region);
+/* Autotuner: times candidate grids (persistent and flat), then per N candidate
+ * region shapes and chunks, on the device (a step reads one buffer and writes the
+ * other, so the state is untouched), and keeps the fastest plan. Blocking. */
 cf_status cf_ensemble_tune(cf_ensemble *e, double dt, int reps, int *grid_out);
 /* n_steps RK4 steps at t_k = t0 + (k0 + k) dt; use_graph: capture (or reuse) a hipGraph.
  * Asynchronous: ordered after earlier work on `stream`, and later work on `stream`

@@ -41,18 +41,9 @@ def make(cfl, prec, lam=0.1):
     return ens
 
 
-@pytest.mark.parametrize("prec,grid", [(0, 0), (1, 0), (0, -1), (1, -1)])
-def test_ensemble_equals_single_lattice_runs(cfl, prec, grid):
-    """Persistent (default grid) and flat (grid = CF_ENS_GRID_FLAT) launches."""
-    print(f"case ensemble prec={prec} grid={grid}", flush=True)
-    ens = make(cfl, prec)
-    if grid:
-        ens.set_plan(0, 0, grid)
-        assert ens.get_plan(NS[0])[1] == grid
-    t0, k0, steps, dt = 1.0, 2, 3, 0.05
-    init = [raw_state(cfl, ens, r) for r in range(ens.n)]
-    ens.run(t0, k0, steps, dt)
-    for r, N in enumerate(NS):
+def check_single_lattice_runs(cfl, ens, prec, init, t0, k0, steps, dt):
+    """Every realisation of `ens` equals the single-lattice run from its initial state."""
+    for r, N in enumerate(ens.Ns):
         p = cfl.kg_params(N, float(N), 1.0, 0.1, cfl.radiation(**FRW))
         if prec == cfl.CF_PREC_F32C:
             hi, lo = init[r]
@@ -65,6 +56,41 @@ def test_ensemble_equals_single_lattice_runs(cfl, prec, grid):
             cfl.rk4_run_f64(p, x, t0, k0, steps, dt, cfl.alloc_work(N, prec))
             (gx,) = raw_state(cfl, ens, r)
             assert_bitwise(host(gx), host(x), f"x r={r}")
+
+
+@pytest.mark.parametrize("prec,grid", [(0, 0), (1, 0), (0, -1), (1, -1)])
+def test_ensemble_equals_single_lattice_runs(cfl, prec, grid):
+    """Persistent (default grid) and flat (grid = CF_ENS_GRID_FLAT) launches."""
+    print(f"case ensemble prec={prec} grid={grid}", flush=True)
+    ens = make(cfl, prec)
+    if grid:
+        ens.set_plan(0, 0, grid)
+        assert ens.get_plan(NS[0])[1] == grid
+    t0, k0, steps, dt = 1.0, 2, 3, 0.05
+    init = [raw_state(cfl, ens, r) for r in range(ens.n)]
+    ens.run(t0, k0, steps, dt)
+    check_single_lattice_runs(cfl, ens, prec, init, t0, k0, steps, dt)
+    ens.close()
+
+
+@pytest.mark.parametrize("prec,grid", [(0, 0), (1, 0), (0, -1), (1, -1)])
+def test_region_shapes_keep_bits(cfl, prec, grid):
+    """Blocks over 24 x 40 regions (16 x 32 tiles: one partial tile per side at N=24
+    and N=40, a single tile at N=16) mixed with 32 x 32 ones in the same launches
+    give every realisation the bits of its single-lattice run."""
+    print(f"case ensemble regions prec={prec} grid={grid}", flush=True)
+    ens = make(cfl, prec)
+    if grid:
+        ens.set_plan(0, 0, grid)
+    for n in (16, 40, 24):
+        ens.set_region(n, cfl.CF_ENS_REGION_24X40)
+        assert ens.get_region(n) == cfl.CF_ENS_REGION_24X40
+    assert ens.get_region(32) == cfl.CF_ENS_REGION_32X32
+    ens.set_plan(40, 16, 0)  # three chunks along a, the last one partial
+    t0, k0, steps, dt = 1.0, 2, 3, 0.05
+    init = [raw_state(cfl, ens, r) for r in range(ens.n)]
+    ens.run(t0, k0, steps, dt)
+    check_single_lattice_runs(cfl, ens, prec, init, t0, k0, steps, dt)
     ens.close()
 
 
