@@ -148,7 +148,7 @@ __device__ __forceinline__ void adj_barrier() {
 // (the order of the running sum s).
 //
 // Per plane j of the march, every thread loads x and lambda' at its own site of
-// plane j+2 (so two planes of loads are in flight) and
+// plane j+2 (used one iteration later, so the load overlaps a whole iteration) and
 //   A(j) on the region minus its outer ring: half A at plane j. The in-plane
 //        neighbours' x.phi, y1.phi, a4.pi and a3.pi come from the LDS slot of plane
 //        j (written by their owners one iteration earlier), the a-neighbours from the
