@@ -342,7 +342,8 @@ def adjoint_workload(args):
     """Loss L = sum_s w_s P_rho(s, t_end), w = 1/(1+k^2), and its gradient with
     respect to phi(t0) and m through 1024 FP64 RK4 steps of KG-FRW at N=128, L=128,
     m=1, dt=0.05 (seed 3, rms 0.3), checkpoints every 32 steps: the forward pass,
-    the loss adjoint and the checkpointed backward pass (each segment recomputed).
+    the loss adjoint and the checkpointed backward pass (every segment but the last
+    recomputed from its checkpoint).
     Reported as wall time per gradient and as forward-equivalent site-steps/s."""
     import torch
     import cfl_amd as cfl
