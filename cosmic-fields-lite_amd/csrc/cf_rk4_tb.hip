@@ -20,36 +20,46 @@ namespace cf {
 
 struct TbGeom {
   int64_t N;
-  int T;      // output tile edge
-  int nt;     // tiles per dimension
-  int A;      // planes per chunk
-  int n_ac;   // chunks
-  int total;  // nt * nt * n_ac
+  int Tb, Tc;    // output tile edges along b and c
+  int ntb, ntc;  // tiles along b and c
+  int A;         // planes per chunk
+  int n_ac;      // chunks
+  int total;     // ntb * ntc * n_ac
 };
 
-template <class T, bool F32C, bool LAM>
+// Work unit of logical block `bid`: tile (tb, tc), chunk ac, c fastest.
+__device__ __forceinline__ TbUnit tb_unit_of(const TbGeom &g, int bid) {
+  const int tc = bid % g.ntc;
+  const int rest = bid / g.ntc;
+  const int tb = rest % g.ntb;
+  const int ac = rest / g.ntb;
+  TbUnit u;
+  u.N = g.N;
+  u.T = g.Tb;
+  u.Tc = g.Tc;
+  u.b0 = (int64_t)tb * g.Tb;
+  u.c0 = (int64_t)tc * g.Tc;
+  u.a0 = (int64_t)ac * g.A;
+  u.a1 = (u.a0 + g.A < g.N) ? u.a0 + g.A : g.N;
+  return u;
+}
+
+// Region shape (b rows x c columns) of the FP32-comp single-lattice kernel. Measured
+// at N=512 (GSite*step/s, same bits): 32 x 32 (24 x 24 tiles) 87.8-88.3; 24 x 40 (16 x 32
+// tiles, whose c edges fall on 128-byte lines; 960 threads) 83.0-83.4; 25 x 40 81.4-82.1.
+constexpr int TB_F32C_RB = 32, TB_F32C_RC = 32;
+
+template <class T, bool F32C, bool LAM, int RB = TBR, int RC = TBR>
 __global__ __launch_bounds__(1024) void k_rk4_tb(TbGeom g, TbStep k, const T *__restrict__ xin,
                                                  const float *__restrict__ lin, T *__restrict__ xout,
                                                  float *__restrict__ lout) {
   __shared__ TbLds<T> lds;
   // Blocks go to the 8 XCDs round robin; consecutive logical tiles are mapped
   // to one XCD so that neighbouring tiles' halos meet in its L2.
-  const int bid = tb_xcd_order(blockIdx.x, g.total);
-  const int tc = bid % g.nt;
-  const int rest = bid / g.nt;
-  const int tb = rest % g.nt;
-  const int ac = rest / g.nt;
-  TbUnit u;
-  u.N = g.N;
-  u.T = g.T;
-  u.Tc = g.T;
-  u.b0 = (int64_t)tb * g.T;
-  u.c0 = (int64_t)tc * g.T;
-  u.a0 = (int64_t)ac * g.A;
-  u.a1 = (u.a0 + g.A < g.N) ? u.a0 + g.A : g.N;
+  const TbUnit u = tb_unit_of(g, tb_xcd_order(blockIdx.x, g.total));
   // Lane shifts for the c neighbours in FP64; LDS in the compensated mode, whose
   // float plane buffers are cheap and whose VALU load is the larger (measured).
-  tb_unit<T, F32C, LAM, true, !F32C>(lds, u, k, xin, lin, xout, lout);
+  tb_unit<T, F32C, LAM, true, !F32C, TB_KG, false, RB, RC>(lds, u, k, xin, lin, xout, lout);
 }
 
 // FP64: the same step with the state streamed into an LDS ring TB_RING_DEPTH planes
@@ -60,19 +70,7 @@ template <bool LAM>
 __global__ __launch_bounds__(1024) void k_rk4_tb_ring(TbGeom g, TbStep k, const double *__restrict__ xin,
                                                       double *__restrict__ xout) {
   __shared__ __attribute__((aligned(16))) char smem[TbRing<TB_RING_DEPTH>::BYTES];
-  const int bid = tb_xcd_order(blockIdx.x, g.total);
-  const int tc = bid % g.nt;
-  const int rest = bid / g.nt;
-  const int tb = rest % g.nt;
-  const int ac = rest / g.nt;
-  TbUnit u;
-  u.N = g.N;
-  u.T = g.T;
-  u.Tc = g.T;
-  u.b0 = (int64_t)tb * g.T;
-  u.c0 = (int64_t)tc * g.T;
-  u.a0 = (int64_t)ac * g.A;
-  u.a1 = (u.a0 + g.A < g.N) ? u.a0 + g.A : g.N;
+  const TbUnit u = tb_unit_of(g, tb_xcd_order(blockIdx.x, g.total));
   tb_unit_ring<LAM, TB_RING_DEPTH>(smem, u, k, xin, xout);
 }
 
@@ -83,30 +81,21 @@ __global__ __launch_bounds__(1024) void k_rk4_tb_var(TbGeom g, TbStep k, const d
                                                      double *__restrict__ xout, const double *__restrict__ psi,
                                                      TbPsi cps) {
   __shared__ TbLds<double> lds;
-  const int bid = tb_xcd_order(blockIdx.x, g.total);
-  const int tc = bid % g.nt;
-  const int rest = bid / g.nt;
-  const int tb = rest % g.nt;
-  const int ac = rest / g.nt;
-  TbUnit u;
-  u.N = g.N;
-  u.T = g.T;
-  u.Tc = g.T;
-  u.b0 = (int64_t)tb * g.T;
-  u.c0 = (int64_t)tc * g.T;
-  u.a0 = (int64_t)ac * g.A;
-  u.a1 = (u.a0 + g.A < g.N) ? u.a0 + g.A : g.N;
+  const TbUnit u = tb_unit_of(g, tb_xcd_order(blockIdx.x, g.total));
   tb_unit<double, false, false, true, true, VAR, true>(lds, u, k, xin, nullptr, xout, nullptr, psi, cps);
 }
 
-static TbGeom tb_geom(int64_t N, int A) {
+// Tiles of (RB - 8) x (RC - 8) sites (the whole extent when N is smaller).
+static TbGeom tb_geom(int64_t N, int A, int RB = TBR, int RC = TBR) {
   TbGeom g;
   g.N = N;
-  g.T = tb_tile(N);
-  g.nt = (int)cdiv(N, g.T);
+  g.Tb = N < RB - 2 * TBH ? (int)N : RB - 2 * TBH;
+  g.Tc = N < RC - 2 * TBH ? (int)N : RC - 2 * TBH;
+  g.ntb = (int)cdiv(N, g.Tb);
+  g.ntc = (int)cdiv(N, g.Tc);
   g.A = (A <= 0 || A > N) ? (int)N : A;
   g.n_ac = (int)cdiv(N, g.A);
-  g.total = g.nt * g.nt * g.n_ac;
+  g.total = g.ntb * g.ntc * g.n_ac;
   return g;
 }
 
@@ -119,12 +108,13 @@ static TbGeom tb_geom(int64_t N, int A) {
 // 128 103 / 89 against 65 / 51; N=384 one chunk 118 / 88 against 92 / 74 with 256;
 // N=512 chunk 256 (the rule's pick) and 171 within 1-3% of each other, one chunk
 // 96 / 77.
-static int tb_default_chunk(int64_t N) {
+static int tb_default_chunk(int64_t N, int RB = TBR, int RC = TBR) {
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
-  const int64_t nt = cdiv(N, tb_tile(N)), tiles = nt * nt;
+  const TbGeom g1 = tb_geom(N, 0, RB, RC);
+  const int64_t tiles = (int64_t)g1.ntb * g1.ntc;
   const int64_t kmax = N / 8 > 1 ? N / 8 : 1;
   std::vector<int64_t> cost((size_t)kmax + 1, -1);
   int64_t best = -1;
@@ -141,6 +131,11 @@ static int tb_default_chunk(int64_t N) {
   return (int)cdiv(N, pick);
 }
 
+// Geometry of the FP32-comp kernel (its region shape), chunk <= 0: the cost model's.
+static TbGeom tb_f32c_geom(int64_t N, int chunk) {
+  return tb_geom(N, chunk > 0 ? chunk : tb_default_chunk(N, TB_F32C_RB, TB_F32C_RC), TB_F32C_RB, TB_F32C_RC);
+}
+
 template <class T, bool F32C>
 static cf_status tb_launch(const TbGeom &g, const TbStep &k, const T *xin, const float *lin, T *xout,
                            float *lout, hipStream_t st) {
@@ -149,13 +144,14 @@ static cf_status tb_launch(const TbGeom &g, const TbStep &k, const T *xin, const
       hipLaunchKernelGGL(k_rk4_tb_ring<true>, dim3(g.total), dim3(1024), 0, st, g, k, xin, xout);
     else
       hipLaunchKernelGGL(k_rk4_tb_ring<false>, dim3(g.total), dim3(1024), 0, st, g, k, xin, xout);
-    CF_LAUNCHED();
-    return CF_OK;
+  } else {
+    if (k.lam != 0.0)
+      hipLaunchKernelGGL((k_rk4_tb<T, F32C, true, TB_F32C_RB, TB_F32C_RC>), dim3(g.total), dim3(1024), 0, st, g, k,
+                         xin, lin, xout, lout);
+    else
+      hipLaunchKernelGGL((k_rk4_tb<T, F32C, false, TB_F32C_RB, TB_F32C_RC>), dim3(g.total), dim3(1024), 0, st, g,
+                         k, xin, lin, xout, lout);
   }
-  if (k.lam != 0.0)
-    hipLaunchKernelGGL((k_rk4_tb<T, F32C, true>), dim3(g.total), dim3(1024), 0, st, g, k, xin, lin, xout, lout);
-  else
-    hipLaunchKernelGGL((k_rk4_tb<T, F32C, false>), dim3(g.total), dim3(1024), 0, st, g, k, xin, lin, xout, lout);
   CF_LAUNCHED();
   return CF_OK;
 }
@@ -291,7 +287,7 @@ cf_status cf_rk4_run_tb_f32c(const cf_kg_params *p, float *hi, float *lo, double
                  disjoint(lo, n * sizeof(float), work, 2 * n * sizeof(float)),
              "cf_rk4_run_tb_f32c: null or overlapping buffer");
   CF_REQUIRE(n_steps >= 0, "cf_rk4_run_tb_f32c: negative step count");
-  const TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N));
+  const TbGeom g = tb_f32c_geom(p->N, chunk);
   float *his[2] = {hi, (float *)work};
   float *los[2] = {lo, (float *)work + n};
   for (int64_t s = 0; s < n_steps; ++s) {
@@ -324,7 +320,7 @@ cf_status cf_rk4_step_tb_f32c(const cf_kg_params *p, const float *hi_in, const f
   CF_REQUIRE(disjoint(hi_in, lo_in, b) && disjoint(hi_out, lo_out, b) && disjoint(hi_in, hi_out, b) &&
                  disjoint(hi_in, lo_out, b) && disjoint(lo_in, hi_out, b) && disjoint(lo_in, lo_out, b),
              "cf_rk4_step_tb_f32c: overlapping buffers");
-  const TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N));
+  const TbGeom g = tb_f32c_geom(p->N, chunk);
   return tb_launch<float, true>(g, tb_step_host(p, t, dt), hi_in, lo_in, hi_out, lo_out, S(stream));
 }
 
