@@ -22,6 +22,7 @@ host on a bounded sample.
     python bench.py --workload ensemble --steps 500       # 96 uneven-N realisations
     python bench.py --workload adjoint                    # loss + gradient, N=128, 1024 steps
     python bench.py --workload variants --steps 10        # fixed-curvature / sqrt-potential steps
+    python bench.py --workload slab                       # one GPU's slab of N=512 (64 planes + ghosts)
 """
 import argparse
 import json
@@ -102,7 +103,7 @@ def parse():
                     help="headline: kernels to time (tb_f32c, tb_f64: one temporally blocked kernel per step; "
                          "f32c, f64: four fused stage kernels per step)")
     ap.add_argument("--save-dir", default="", help="ensemble: where to save the averaged density spectra (.npy)")
-    ap.add_argument("--workload", choices=("headline", "ensemble", "precision", "adjoint", "variants"),
+    ap.add_argument("--workload", choices=("headline", "ensemble", "precision", "adjoint", "variants", "slab"),
                     default="headline",
                     help="headline: N=512 FRW, FP32-comp vs FP64; ensemble: 96 realisations of N=64/96/128; "
                          "precision: run A (FP32-comp) vs run B (FP64) errors every 100 steps; "
@@ -378,6 +379,49 @@ def adjoint_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def slab_workload(args):
+    """One GPU's share of a slab decomposition along a: the blocked FP64 step on 64 of the
+    512 planes of the headline lattice (KG-FRW N=512, eight GPUs' worth of slabs), with
+    four ghost planes on each side taken from the whole lattice once (their refresh
+    between steps is the exchange, not timed here). One launch per step, ping-ponging
+    two slab buffers; HIP events on the launch stream."""
+    import torch
+    import cfl_amd as cfl
+    N, G = args.n, 8
+    n = N // G
+    L, dt, t0 = float(N), 0.05, 1.0
+    p = cfl.kg_params(N, L, 1.0, 0.0, cfl.radiation(1.0, 0.5, 1.0))
+    plan = cfl.SpectrumPlan(N)
+    x0 = plan.init_grf_state(2, cfl.CF_SPEC_GAUSS, (0.25,), L, 1.0)
+    del plan
+    idx = torch.tensor([(i - 4) % N for i in range(n + 8)], device="cuda")
+    slab = torch.cat([x0[: N ** 3].view(N, N * N)[idx].reshape(-1), x0[N ** 3:].view(N, N * N)[idx].reshape(-1)])
+    del x0
+    bufs = [slab, slab.clone()]
+    for k in range(args.warmup):
+        cfl.rk4_step_tb_f64_slab(p, n, bufs[k & 1], bufs[(k + 1) & 1], t0 + k * dt, dt)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for k in range(args.warmup, args.warmup + args.steps):
+        cfl.rk4_step_tb_f64_slab(p, n, bufs[k & 1], bufs[(k + 1) & 1], t0 + k * dt, dt)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    sites = N * N * n
+    line = {
+        "metric": "GSite*RK4-step/s of one GPU's slab (FP64, temporally blocked)",
+        "value": sites / (ms * 1e-3) / 1e9, "unit": "GSite*RK4-step/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "dtype": "f64",
+        "data": "synthetic (Gaussian random field, seed 2)",
+        "config": {"workload": f"KG-FRW N={N}^3 L={L:g} m=1 dt=0.05, slab of {n} planes + 2 x 4 ghost planes "
+                               f"(one of {G} slabs along a)", "slab_planes": n},
+        "ghost_bytes_per_neighbour_per_step": 4 * N * N * 16,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def variants_workload(args):
     """FP64 RK4 steps of the fixed-curvature (Psi: a Gaussian random field, rms 0.1)
     and sqrt-potential (f_a = 0.7) equations at N=512, L=512, m=1.1, radiation FRW,
@@ -462,6 +506,9 @@ def main():
         return
     if args.workload == "adjoint":
         adjoint_workload(args)
+        return
+    if args.workload == "slab":
+        slab_workload(args)
         return
     if args.workload == "precision":
         precision_workload(args)

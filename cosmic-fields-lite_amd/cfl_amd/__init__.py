@@ -158,6 +158,14 @@ def rk4_step_tb_f64(p, x_in, x_out, t, dt, chunk=0, stream=None):
     check(lib.cf_rk4_step_tb_f64(C.byref(p), _ptr(x_in), _ptr(x_out), t, dt, chunk, _stream(stream)))
 
 
+def rk4_step_tb_f64_slab(p, n_planes, x_in, x_out, t, dt, chunk=0, stream=None):
+    """One temporally blocked FP64 step of a slab of n_planes planes with four ghost
+    planes on each side ([phi (n_planes+8) N^2 | pi (n_planes+8) N^2]); writes the
+    slab's own planes of x_out."""
+    check(lib.cf_rk4_step_tb_f64_slab(C.byref(p), n_planes, _ptr(x_in), _ptr(x_out), t, dt, chunk,
+                                      _stream(stream)))
+
+
 def rk4_step_tb_f32c(p, hi_in, lo_in, hi_out, lo_out, t, dt, chunk=0, stream=None):
     check(lib.cf_rk4_step_tb_f32c(C.byref(p), _ptr(hi_in), _ptr(lo_in), _ptr(hi_out), _ptr(lo_out), t, dt, chunk,
                                   _stream(stream)))

@@ -20,11 +20,13 @@ namespace cf {
 
 struct TbGeom {
   int64_t N;
-  int Tb, Tc;    // output tile edges along b and c
-  int ntb, ntc;  // tiles along b and c
-  int A;         // planes per chunk
-  int n_ac;      // chunks
-  int total;     // ntb * ntc * n_ac
+  int Tb, Tc;        // output tile edges along b and c
+  int ntb, ntc;      // tiles along b and c
+  int A;             // planes per chunk
+  int n_ac;          // chunks
+  int total;         // ntb * ntc * n_ac
+  int64_t NA;        // planes of the state buffers along a (N, or a slab's planes + ghosts)
+  int64_t a_lo, a_hi;  // output planes [a_lo, a_hi)
 };
 
 // Work unit of logical block `bid`: tile (tb, tc), chunk ac, c fastest.
@@ -39,8 +41,9 @@ __device__ __forceinline__ TbUnit tb_unit_of(const TbGeom &g, int bid) {
   u.Tc = g.Tc;
   u.b0 = (int64_t)tb * g.Tb;
   u.c0 = (int64_t)tc * g.Tc;
-  u.a0 = (int64_t)ac * g.A;
-  u.a1 = (u.a0 + g.A < g.N) ? u.a0 + g.A : g.N;
+  u.a0 = g.a_lo + (int64_t)ac * g.A;
+  u.a1 = (u.a0 + g.A < g.a_hi) ? u.a0 + g.A : g.a_hi;
+  u.NA = g.NA;
   return u;
 }
 
@@ -85,16 +88,22 @@ __global__ __launch_bounds__(1024) void k_rk4_tb_var(TbGeom g, TbStep k, const d
   tb_unit<double, false, false, true, true, VAR, true>(lds, u, k, xin, nullptr, xout, nullptr, psi, cps);
 }
 
-// Tiles of (RB - 8) x (RC - 8) sites (the whole extent when N is smaller).
-static TbGeom tb_geom(int64_t N, int A, int RB = TBR, int RC = TBR) {
+// Tiles of (RB - 8) x (RC - 8) sites (the whole extent when N is smaller) over the
+// output planes [a_lo, a_hi) of buffers of NA planes; a periodic lattice: 0, N, N.
+static TbGeom tb_geom(int64_t N, int A, int RB = TBR, int RC = TBR, int64_t a_lo = 0, int64_t a_hi = -1,
+                      int64_t NA = -1) {
   TbGeom g;
   g.N = N;
+  g.a_lo = a_lo;
+  g.a_hi = a_hi < 0 ? N : a_hi;
+  g.NA = NA < 0 ? N : NA;
+  const int64_t planes = g.a_hi - g.a_lo;
   g.Tb = N < RB - 2 * TBH ? (int)N : RB - 2 * TBH;
   g.Tc = N < RC - 2 * TBH ? (int)N : RC - 2 * TBH;
   g.ntb = (int)cdiv(N, g.Tb);
   g.ntc = (int)cdiv(N, g.Tc);
-  g.A = (A <= 0 || A > N) ? (int)N : A;
-  g.n_ac = (int)cdiv(N, g.A);
+  g.A = (A <= 0 || A > planes) ? (int)planes : A;
+  g.n_ac = (int)cdiv(planes, g.A);
   g.total = g.ntb * g.ntc * g.n_ac;
   return g;
 }
@@ -108,19 +117,20 @@ static TbGeom tb_geom(int64_t N, int A, int RB = TBR, int RC = TBR) {
 // 128 103 / 89 against 65 / 51; N=384 one chunk 118 / 88 against 92 / 74 with 256;
 // N=512 chunk 256 (the rule's pick) and 171 within 1-3% of each other, one chunk
 // 96 / 77.
-static int tb_default_chunk(int64_t N, int RB = TBR, int RC = TBR) {
+static int tb_default_chunk(int64_t N, int RB = TBR, int RC = TBR, int64_t planes = -1) {
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
+  if (planes < 0) planes = N;  // output planes of the launch
   const TbGeom g1 = tb_geom(N, 0, RB, RC);
   const int64_t tiles = (int64_t)g1.ntb * g1.ntc;
-  const int64_t kmax = N / 8 > 1 ? N / 8 : 1;
+  const int64_t kmax = planes / 8 > 1 ? planes / 8 : 1;
   std::vector<int64_t> cost((size_t)kmax + 1, -1);
   int64_t best = -1;
   for (int64_t k = 1; k <= kmax; ++k) {
-    const int64_t ch = cdiv(N, k);
-    const int64_t kk = cdiv(N, ch);  // chunks actually launched for this chunk length
+    const int64_t ch = cdiv(planes, k);
+    const int64_t kk = cdiv(planes, ch);  // chunks actually launched for this chunk length
     if (kk != k) continue;
     cost[k] = cdiv(tiles * k, cus) * (ch + 8);
     if (best < 0 || cost[k] < best) best = cost[k];
@@ -128,7 +138,7 @@ static int tb_default_chunk(int64_t N, int RB = TBR, int RC = TBR) {
   int64_t pick = 1;
   for (int64_t k = 1; k <= kmax; ++k)
     if (cost[k] >= 0 && cost[k] * 100 <= best * 102) pick = k;
-  return (int)cdiv(N, pick);
+  return (int)cdiv(planes, pick);
 }
 
 // Geometry of the FP32-comp kernel (its region shape), chunk <= 0: the cost model's.
@@ -309,6 +319,21 @@ cf_status cf_rk4_step_tb_f64(const cf_kg_params *p, const double *x_in, double *
   CF_REQUIRE(x_in && x_out && disjoint(x_in, x_out, n * sizeof(double)),
              "cf_rk4_step_tb_f64: null or overlapping state buffers");
   const TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N));
+  return tb_launch<double, false>(g, tb_step_host(p, t, dt), x_in, nullptr, x_out, nullptr, S(stream));
+}
+
+cf_status cf_rk4_step_tb_f64_slab(const cf_kg_params *p, int64_t n_planes, const double *x_in, double *x_out,
+                                  double t, double dt, int chunk, cf_stream stream) {
+  CF_TRY(tb_check(p));
+  CF_REQUIRE(n_planes >= 1 && n_planes <= p->N, "cf_rk4_step_tb_f64_slab: n_planes=%lld outside 1..N",
+             (long long)n_planes);
+  const int64_t NA = n_planes + 2 * TBH;  // own planes and four ghost planes on each side
+  const size_t bytes = 2 * (size_t)NA * p->N * p->N * sizeof(double);
+  CF_REQUIRE(x_in && x_out && disjoint(x_in, x_out, bytes), "cf_rk4_step_tb_f64_slab: null or overlapping slabs");
+  // A unit reads planes a0 - 4 .. a1 + 3 of its chunk (wrapping at NA only in the ring's
+  // dead prefetches past a1 + 3), all inside the slab for chunks within [4, 4 + n_planes).
+  const TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N, TBR, TBR, n_planes), TBR, TBR, TBH,
+                           TBH + n_planes, NA);
   return tb_launch<double, false>(g, tb_step_host(p, t, dt), x_in, nullptr, x_out, nullptr, S(stream));
 }
 

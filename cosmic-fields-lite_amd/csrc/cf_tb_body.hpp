@@ -67,11 +67,14 @@ struct TbPsi {
 };
 
 // One work unit: lattice size, output tile origin and edges (T along b, Tc along
-// c), plane chunk.
+// c), plane chunk, and the planes NA that the state buffers hold along a (N for a
+// periodic lattice; a slab of a larger lattice holds its planes plus ghost planes,
+// and the unit's planes stay within it). Plane indices wrap at NA.
 struct TbUnit {
   int64_t N;
   int64_t b0, c0, a0, a1;
   int T, Tc;
+  int64_t NA;
 };
 
 __device__ __forceinline__ double tb_rhs(double phi, double pa1, double pam, double pb1, double pbm, double pc1,
@@ -215,7 +218,7 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds_all, const TbUnit &u, cons
   // increment), so the histories are held as float (half the registers) and
   // widened where they are used; the arithmetic stays double.
   typedef std::conditional_t<F32C, float, double> Hv;
-  const int64_t N = u.N, NN = N * N, N3 = NN * N;
+  const int64_t N = u.N, NN = N * N, NA = u.NA, N3 = NN * NA;  // N3: offset of the pi planes
   const int plane_bytes = (int)(NN * (int64_t)sizeof(T)), plane_bytes_f = (int)(NN * (int64_t)sizeof(float));
   int64_t b = (u.b0 - TBH + r) % N;
   if (b < 0) b += N;
@@ -240,8 +243,8 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds_all, const TbUnit &u, cons
 
   // plane offsets (times N^2) advanced by one plane per iteration, wrapping at N
   auto wrap_plane = [&](int64_t P) -> int64_t {
-    int64_t q = P % N;
-    if (q < 0) q += N;
+    int64_t q = P % NA;
+    if (q < 0) q += NA;
     return q * NN;
   };
   auto next_plane = [&](int64_t off) -> int64_t {
@@ -444,7 +447,7 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds_all, const TbUnit &u, cons
     q.plane = off_st;
     if constexpr (!DEFER) flush(q);
     off_st = next_plane(off_st);
-    a_st = (a_st + 1 == N) ? 0 : a_st + 1;
+    a_st = (a_st + 1 == NA) ? 0 : a_st + 1;
   };
   using std::integral_constant;
   if constexpr (RING) {

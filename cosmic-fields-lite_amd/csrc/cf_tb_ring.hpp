@@ -59,7 +59,7 @@ __device__ __forceinline__ void tb_unit_ring(char *smem, const TbUnit &u, const 
   const int tid = threadIdx.x;
   const int r = tid >> 5, cc = tid & 31;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6), ln = tid & 63;  // wavefront index (uniform)
-  const int64_t N = u.N, NN = N * N, N3 = NN * N;
+  const int64_t N = u.N, NN = N * N, NA = u.NA, N3 = NN * NA;  // N3: offset of the pi planes
   const int plane_bytes = (int)(NN * (int64_t)sizeof(double));
   int64_t b = (u.b0 - TBH + r) % N;
   if (b < 0) b += N;
@@ -83,8 +83,8 @@ __device__ __forceinline__ void tb_unit_ring(char *smem, const TbUnit &u, const 
   const int lds_wave = comp * TBR * TBR + 4 * (w & 7) * TBR;  // first element this wavefront writes in a slot
 
   auto wrap_plane = [&](int64_t P) -> int64_t {
-    int64_t q = P % N;
-    if (q < 0) q += N;
+    int64_t q = P % NA;
+    if (q < 0) q += NA;
     return q * NN;
   };
   int64_t pl_ld = wrap_plane(u.a0 - TBH);  // next plane to load

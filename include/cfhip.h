@@ -164,6 +164,19 @@ cf_status cf_rk4_step_tb_f64(const cf_kg_params *p, const double *x_in_dev, doub
 cf_status cf_rk4_step_tb_f32c(const cf_kg_params *p, const float *hi_in_dev, const float *lo_in_dev,
                               float *hi_out_dev, float *lo_out_dev, double t, double dt, int chunk,
                               cf_stream stream);
+/* One temporally blocked FP64 step of a slab of an N^3 lattice split along a: the
+ * share of one GPU in a slab decomposition (no reference counterpart; the reference
+ * runs one lattice on one device). x_in_dev and x_out_dev hold n_planes + 8 planes of
+ * phi, then as many of pi: [phi (n_planes + 8) N^2 | pi (n_planes + 8) N^2]. Planes
+ * 4 .. n_planes + 3 are the slab's own (lattice planes a0 .. a0 + n_planes - 1);
+ * planes 0..3 and n_planes + 4 .. n_planes + 7 are ghost copies of the neighbours'
+ * planes a0 - 4 .. a0 - 1 and a0 + n_planes .. a0 + n_planes + 3 (periodic), filled
+ * before the step. Writes the slab's own planes of x_out_dev and leaves its ghost
+ * planes as they were; b and c are periodic as usual. Each own site gets the bits of
+ * cf_rk4_step_tb_f64 on the whole lattice. n_planes >= 1; the buffers must not
+ * overlap. */
+cf_status cf_rk4_step_tb_f64_slab(const cf_kg_params *p, int64_t n_planes, const double *x_in_dev,
+                                  double *x_out_dev, double t, double dt, int chunk, cf_stream stream);
 
 /* Split a double state into hi/lo floats and back (x = hi + lo). */
 cf_status cf_split_f64(const double *x_dev, float *hi_dev, float *lo_dev, int64_t n, cf_stream stream);

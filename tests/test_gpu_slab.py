@@ -1,0 +1,94 @@
+"""The blocked FP64 step on a slab of the lattice (the share of one GPU in a slab
+decomposition along a): with its four ghost planes on each side copied from the
+periodic neighbours, every own site equals the whole-lattice step bit for bit, and
+the ghost planes of the output are left alone."""
+import numpy as np
+import pytest
+import torch
+
+from test_gpu_parity import assert_bitwise, host
+
+pytestmark = pytest.mark.gpu
+
+FRW = dict(a1=1.0, H1=0.5, t1=1.0)
+
+
+def state(cfl, N, seed):
+    rng = np.random.default_rng(seed)
+    x = np.concatenate([rng.standard_normal(N ** 3), 0.3 * rng.standard_normal(N ** 3)])
+    return torch.from_numpy(x).cuda()
+
+
+def slab_of(x, N, a0, n):
+    """Planes a0 - 4 .. a0 + n + 3 (periodic) of phi, then of pi."""
+    idx = torch.tensor([(a0 - 4 + i) % N for i in range(n + 8)], device=x.device)
+    phi = x[: N ** 3].view(N, N * N)[idx]
+    pi = x[N ** 3:].view(N, N * N)[idx]
+    return torch.cat([phi.reshape(-1), pi.reshape(-1)])
+
+
+def check_split(cfl, N, sizes, lam=0.0, chunk=0, t=1.3, dt=0.05):
+    print(f"case slab N={N} sizes={sizes} lam={lam} chunk={chunk}", flush=True)
+    assert sum(sizes) == N
+    p = cfl.kg_params(N, float(N), 1.0, lam, cfl.radiation(**FRW))
+    x = state(cfl, N, N + len(sizes))
+    whole = torch.empty_like(x)
+    cfl.rk4_step_tb_f64(p, x, whole, t, dt)
+    ref = host(whole)
+    a0 = 0
+    for n in sizes:
+        xin = slab_of(x, N, a0, n)
+        xout = torch.full_like(xin, float("nan"))
+        cfl.rk4_step_tb_f64_slab(p, n, xin, xout, t, dt, chunk)
+        out = host(xout).reshape(2, n + 8, N * N)
+        for comp in range(2):
+            own = out[comp, 4:4 + n].reshape(-1)
+            exp = ref[comp * N ** 3:(comp + 1) * N ** 3].reshape(N, N * N)[a0:a0 + n].reshape(-1)
+            assert_bitwise(own, exp, f"slab a0={a0} n={n} comp={comp}")
+            assert np.isnan(out[comp, :4]).all() and np.isnan(out[comp, 4 + n:]).all(), "ghost planes written"
+        a0 += n
+
+
+def test_slab_even_split(cfl):
+    check_split(cfl, 64, [16, 16, 16, 16])
+
+
+def test_slab_uneven_split_partial_tiles_lambda(cfl):
+    # N=40: one 24-column tile and one of 16 per side; slabs of 13, 1, 26 planes
+    check_split(cfl, 40, [13, 1, 26], lam=0.1)
+
+
+def test_slab_explicit_chunk(cfl):
+    check_split(cfl, 32, [20, 12], chunk=3)
+
+
+def test_slab_one_gpu_share_n512(cfl):
+    """N=512 split for 8 GPUs: the last slab's upper ghost planes wrap to planes 0..3."""
+    N, n = 512, 64
+    print("case slab N=512 share of 8", flush=True)
+    p = cfl.kg_params(N, float(N), 1.0, 0.0, cfl.radiation(**FRW))
+    x = state(cfl, N, 5)
+    whole = torch.empty_like(x)
+    cfl.rk4_step_tb_f64(p, x, whole, 1.0, 0.05)
+    for a0 in (0, N - n):
+        xin = slab_of(x, N, a0, n)
+        xout = torch.full_like(xin, float("nan"))
+        cfl.rk4_step_tb_f64_slab(p, n, xin, xout, 1.0, 0.05)
+        for comp in range(2):
+            own = xout.view(2, n + 8, N * N)[comp, 4:4 + n]
+            exp = whole[comp * N ** 3:(comp + 1) * N ** 3].view(N, N * N)[a0:a0 + n]
+            assert torch.equal(own, exp), f"a0={a0} comp={comp}"
+        del xin, xout
+    torch.cuda.synchronize()
+
+
+def test_slab_rejects_bad_sizes(cfl):
+    p = cfl.kg_params(16, 16.0, 1.0, 0.0, cfl.radiation(**FRW))
+    buf = torch.zeros(2 * 9 * 256, dtype=torch.float64, device="cuda")
+    out = torch.zeros_like(buf)
+    with pytest.raises(cfl.CfError):
+        cfl.rk4_step_tb_f64_slab(p, 0, buf, out, 1.0, 0.05)
+    with pytest.raises(cfl.CfError):
+        cfl.rk4_step_tb_f64_slab(p, 17, buf, out, 1.0, 0.05)
+    with pytest.raises(cfl.CfError):
+        cfl.rk4_step_tb_f64_slab(p, 1, buf, buf, 1.0, 0.05)
