@@ -52,7 +52,7 @@ __device__ __forceinline__ TbUnit tb_unit_of(const TbGeom &g, int bid) {
 // tiles, whose c edges fall on 128-byte lines; 960 threads) 83.0-83.4; 25 x 40 81.4-82.1.
 constexpr int TB_F32C_RB = 32, TB_F32C_RC = 32;
 
-template <class T, bool F32C, bool LAM, int RB = TBR, int RC = TBR>
+template <class T, bool F32C, bool LAM>
 __global__ __launch_bounds__(1024) void k_rk4_tb(TbGeom g, TbStep k, const T *__restrict__ xin,
                                                  const float *__restrict__ lin, T *__restrict__ xout,
                                                  float *__restrict__ lout) {
@@ -62,7 +62,8 @@ __global__ __launch_bounds__(1024) void k_rk4_tb(TbGeom g, TbStep k, const T *__
   const TbUnit u = tb_unit_of(g, tb_xcd_order(blockIdx.x, g.total));
   // Lane shifts for the c neighbours in FP64; LDS in the compensated mode, whose
   // float plane buffers are cheap and whose VALU load is the larger (measured).
-  tb_unit<T, F32C, LAM, true, !F32C, TB_KG, false, RB, RC>(lds, u, k, xin, lin, xout, lout);
+  constexpr int RB = F32C ? TB_F32C_RB : TBR, RC = F32C ? TB_F32C_RC : TBR;
+  tb_unit<T, F32C, LAM, true, !F32C && RC == TBR, TB_KG, false, RB, RC>(lds, u, k, xin, lin, xout, lout);
 }
 
 // FP64: the same step with the state streamed into an LDS ring TB_RING_DEPTH planes
@@ -156,11 +157,9 @@ static cf_status tb_launch(const TbGeom &g, const TbStep &k, const T *xin, const
       hipLaunchKernelGGL(k_rk4_tb_ring<false>, dim3(g.total), dim3(1024), 0, st, g, k, xin, xout);
   } else {
     if (k.lam != 0.0)
-      hipLaunchKernelGGL((k_rk4_tb<T, F32C, true, TB_F32C_RB, TB_F32C_RC>), dim3(g.total), dim3(1024), 0, st, g, k,
-                         xin, lin, xout, lout);
+      hipLaunchKernelGGL((k_rk4_tb<T, F32C, true>), dim3(g.total), dim3(1024), 0, st, g, k, xin, lin, xout, lout);
     else
-      hipLaunchKernelGGL((k_rk4_tb<T, F32C, false, TB_F32C_RB, TB_F32C_RC>), dim3(g.total), dim3(1024), 0, st, g,
-                         k, xin, lin, xout, lout);
+      hipLaunchKernelGGL((k_rk4_tb<T, F32C, false>), dim3(g.total), dim3(1024), 0, st, g, k, xin, lin, xout, lout);
   }
   CF_LAUNCHED();
   return CF_OK;
