@@ -128,6 +128,18 @@ def test_bad_arguments_rejected_before_device_work(lib):
     Ls = (C.c_double * 2)(16.0, 7.0)
     assert L.cf_ensemble_create(2, Ns, Ls, 1.0, 0.0, C.byref(lib.Cosmology(1, 0, 0, 1)), 0, C.byref(h)) == \
         lib.CF_E_INVALID  # odd N
+    r = C.c_int()
+    assert L.cf_ensemble_set_region(None, 16, lib.CF_ENS_REGION_24X40) == lib.CF_E_INVALID  # null handle
+    assert L.cf_ensemble_get_region(None, 16, C.byref(r)) == lib.CF_E_INVALID
+    # slab step: plane count outside 1..N, overlapping slabs (before any device work)
+    assert L.cf_rk4_step_tb_f64_slab(C.byref(p), 0, x0, C.c_void_p(0x40000000), 1.0, 0.1, 0, None) == \
+        lib.CF_E_INVALID
+    assert L.cf_rk4_step_tb_f64_slab(C.byref(p), 17, x0, C.c_void_p(0x40000000), 1.0, 0.1, 0, None) == \
+        lib.CF_E_INVALID
+    assert b"n_planes" in L.cf_last_error()
+    assert L.cf_rk4_step_tb_f64_slab(C.byref(p), 4, x0, C.c_void_p(0x100000 + 8 * 2 * 12 * 256 - 8), 1.0, 0.1, 0,
+                                     None) == lib.CF_E_INVALID  # the slabs overlap by one double
+    assert b"overlapping" in L.cf_last_error()
 
 
 @pytest.mark.parametrize("t_f", [10.0, 37.5, 160.0])
