@@ -166,6 +166,12 @@ def rk4_step_tb_f64_slab(p, n_planes, x_in, x_out, t, dt, chunk=0, stream=None):
                                       _stream(stream)))
 
 
+def rk4_step_tb_f32c_slab(p, n_planes, hi_in, lo_in, hi_out, lo_out, t, dt, chunk=0, stream=None):
+    """The FP32-comp form of rk4_step_tb_f64_slab (hi and lo laid out like its slab)."""
+    check(lib.cf_rk4_step_tb_f32c_slab(C.byref(p), n_planes, _ptr(hi_in), _ptr(lo_in), _ptr(hi_out), _ptr(lo_out),
+                                       t, dt, chunk, _stream(stream)))
+
+
 def rk4_step_tb_f32c(p, hi_in, lo_in, hi_out, lo_out, t, dt, chunk=0, stream=None):
     check(lib.cf_rk4_step_tb_f32c(C.byref(p), _ptr(hi_in), _ptr(lo_in), _ptr(hi_out), _ptr(lo_out), t, dt, chunk,
                                   _stream(stream)))

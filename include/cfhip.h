@@ -177,6 +177,12 @@ cf_status cf_rk4_step_tb_f32c(const cf_kg_params *p, const float *hi_in_dev, con
  * overlap. */
 cf_status cf_rk4_step_tb_f64_slab(const cf_kg_params *p, int64_t n_planes, const double *x_in_dev,
                                   double *x_out_dev, double t, double dt, int chunk, cf_stream stream);
+/* The same for FP32-comp storage: hi and lo each laid out as the FP64 slab above (n_planes
+ * + 8 planes of phi, then of pi). Each own site gets the bits of cf_rk4_step_tb_f32c;
+ * no two of the four buffers may overlap. */
+cf_status cf_rk4_step_tb_f32c_slab(const cf_kg_params *p, int64_t n_planes, const float *hi_in_dev,
+                                   const float *lo_in_dev, float *hi_out_dev, float *lo_out_dev, double t,
+                                   double dt, int chunk, cf_stream stream);
 
 /* Split a double state into hi/lo floats and back (x = hi + lo). */
 cf_status cf_split_f64(const double *x_dev, float *hi_dev, float *lo_dev, int64_t n, cf_stream stream);

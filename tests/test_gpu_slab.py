@@ -82,6 +82,28 @@ def test_slab_one_gpu_share_n512(cfl):
     torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("N,sizes,lam", [(64, [16, 16, 16, 16], 0.0), (40, [13, 1, 26], 0.1)])
+def test_slab_f32c(cfl, N, sizes, lam):
+    """FP32-comp storage: hi and lo slabs, own sites equal the whole-lattice FP32-comp step."""
+    print(f"case slab f32c N={N} sizes={sizes} lam={lam}", flush=True)
+    p = cfl.kg_params(N, float(N), 1.0, lam, cfl.radiation(**FRW))
+    hi, lo = cfl.split_f64(state(cfl, N, 7))
+    whi, wlo = torch.empty_like(hi), torch.empty_like(lo)
+    cfl.rk4_step_tb_f32c(p, hi, lo, whi, wlo, 1.3, 0.05)
+    a0 = 0
+    for n in sizes:
+        shi, slo = slab_of(hi, N, a0, n), slab_of(lo, N, a0, n)
+        ohi = torch.full_like(shi, float("nan"))
+        olo = torch.full_like(slo, float("nan"))
+        cfl.rk4_step_tb_f32c_slab(p, n, shi, slo, ohi, olo, 1.3, 0.05)
+        for got, whole, name in ((ohi, whi, "hi"), (olo, wlo, "lo")):
+            g = got.view(2, n + 8, N * N)
+            w = whole.view(2, N, N * N)
+            assert torch.equal(g[:, 4:4 + n], w[:, a0:a0 + n]), f"{name} a0={a0} n={n}"
+            assert torch.isnan(g[:, :4]).all() and torch.isnan(g[:, 4 + n:]).all(), "ghost planes written"
+        a0 += n
+
+
 def test_slab_rejects_bad_sizes(cfl):
     p = cfl.kg_params(16, 16.0, 1.0, 0.0, cfl.radiation(**FRW))
     buf = torch.zeros(2 * 9 * 256, dtype=torch.float64, device="cuda")
