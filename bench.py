@@ -476,6 +476,7 @@ def variants_workload(args):
     x0 = plan.init_grf_state(2, cfl.CF_SPEC_GAUSS, (0.25,), float(N), 1.0)
     psi = plan.init_grf_state(3, cfl.CF_SPEC_GAUSS, (0.25,), float(N), 0.1)[: N ** 3].clone()
     del plan
+    settle(cfl, p, x0)
     K, W = args.steps, max(1, args.warmup)
     out = {}
     for name, kind, field in (("fixed_curvature", cfl.CF_VAR_FIXED_CURVATURE, psi),
