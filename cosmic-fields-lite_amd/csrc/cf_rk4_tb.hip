@@ -142,6 +142,19 @@ static int tb_default_chunk(int64_t N, int RB = TBR, int RC = TBR, int64_t plane
   return (int)cdiv(planes, pick);
 }
 
+// Where the FP32-comp run puts its second hi and lo arrays in the work buffer: at the
+// first address from `from` on with the same residue modulo TB_ALIGN_SLACK as the
+// caller's array `like`. Measured at N=512: hi, lo and the second pair at relative
+// offsets that are not multiples of 4 KB (one allocation carved 256 B - 2 KB apart)
+// ran 72-83 GSite*step/s against 84-90 at multiples of 4 KB. Matching residues keeps
+// each array's offset to its partner of the other step a multiple of 4 KB; the work
+// buffer carries the slack (cf_rk4_tb_work_bytes).
+constexpr size_t TB_ALIGN_SLACK = 4096;
+static float *same_residue(void *from, const float *like) {
+  const uintptr_t f = reinterpret_cast<uintptr_t>(from), l = reinterpret_cast<uintptr_t>(like);
+  return reinterpret_cast<float *>(f + ((l - f) & (TB_ALIGN_SLACK - 1)));
+}
+
 // Geometry of the FP32-comp kernel (its region shape), chunk <= 0: the cost model's.
 static TbGeom tb_f32c_geom(int64_t N, int chunk) {
   return tb_geom(N, chunk > 0 ? chunk : tb_default_chunk(N, TB_F32C_RB, TB_F32C_RC), TB_F32C_RB, TB_F32C_RC);
@@ -241,7 +254,7 @@ extern "C" {
 size_t cf_rk4_tb_work_bytes(int64_t N, int precision) {
   if (!valid_N(N)) return 0;
   const size_t n = 2 * (size_t)N * N * N;
-  return precision == CF_PREC_F32C ? 2 * n * sizeof(float) : n * sizeof(double);
+  return precision == CF_PREC_F32C ? 2 * n * sizeof(float) + 2 * TB_ALIGN_SLACK : n * sizeof(double);
 }
 
 cf_status cf_rk4_run_tb_f64(const cf_kg_params *p, double *x, double t0, int64_t k0, int64_t n_steps, double dt,
@@ -291,14 +304,14 @@ cf_status cf_rk4_run_tb_f32c(const cf_kg_params *p, float *hi, float *lo, double
                              double dt, int chunk, void *work, cf_stream stream) {
   CF_TRY(tb_check(p));
   const size_t n = 2 * (size_t)p->N * p->N * p->N;
+  const size_t wb = 2 * n * sizeof(float) + 2 * TB_ALIGN_SLACK;  // cf_rk4_tb_work_bytes
   CF_REQUIRE(hi && lo && work && disjoint(hi, lo, n * sizeof(float)) &&
-                 disjoint(hi, n * sizeof(float), work, 2 * n * sizeof(float)) &&
-                 disjoint(lo, n * sizeof(float), work, 2 * n * sizeof(float)),
+                 disjoint(hi, n * sizeof(float), work, wb) && disjoint(lo, n * sizeof(float), work, wb),
              "cf_rk4_run_tb_f32c: null or overlapping buffer");
   CF_REQUIRE(n_steps >= 0, "cf_rk4_run_tb_f32c: negative step count");
   const TbGeom g = tb_f32c_geom(p->N, chunk);
-  float *his[2] = {hi, (float *)work};
-  float *los[2] = {lo, (float *)work + n};
+  float *his[2] = {hi, same_residue(work, hi)};
+  float *los[2] = {lo, same_residue(his[1] + n, lo)};
   for (int64_t s = 0; s < n_steps; ++s) {
     const double t = t0 + static_cast<double>(k0 + s) * dt;
     CF_TRY((tb_launch<float, true>(g, tb_step_host(p, t, dt), his[s & 1], los[s & 1], his[(s + 1) & 1],

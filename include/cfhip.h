@@ -138,8 +138,10 @@ cf_status cf_rk4_run_f32c(const cf_kg_params *p, float *hi_dev, float *lo_dev, d
  * cf_rk4_run_f64 / cf_rk4_run_f32c. Steps ping-pong between the state and
  * work_dev (cf_rk4_tb_work_bytes); the result is always left in the state.
  * chunk = planes per block along a (<= 0: default). FP32-comp speed depends on the
- * relative placement of hi, lo and work: allocate them separately, or at relative
- * offsets that are multiples of 4 KB. */
+ * relative placement of the arrays (up to 20% at N=512): the run places its second hi
+ * and lo inside work at the caller's hi and lo addresses modulo 4 KB (work carries 8 KB
+ * of slack for it), so hi and lo themselves should be allocated separately or at a
+ * relative offset that is a multiple of 4 KB. */
 size_t cf_rk4_tb_work_bytes(int64_t N, int precision);
 cf_status cf_rk4_run_tb_f64(const cf_kg_params *p, double *x_dev, double t0, int64_t k0, int64_t n_steps,
                             double dt, int chunk, void *work_dev, cf_stream stream);

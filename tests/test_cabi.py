@@ -76,7 +76,7 @@ def test_sizes(lib, O):
     assert L.cf_rk4_work_bytes(7, 0) == 0  # odd N unsupported
     # temporally blocked steps ping-pong with one more state (FP32-comp: hi and lo)
     assert L.cf_rk4_tb_work_bytes(512, 0) == 2 * 512 ** 3 * 8
-    assert L.cf_rk4_tb_work_bytes(512, 1) == 2 * 2 * 512 ** 3 * 4
+    assert L.cf_rk4_tb_work_bytes(512, 1) == 2 * 2 * 512 ** 3 * 4 + 2 * 4096  # + slack to align the second pair
     # adjoint: checkpoints + one segment + step work (2) + lambda (2) + forward work
     # (3) state vectors, and 4 dL/dm partials per block (512 blocks) per step plus one
     v = 2 * 128 ** 3 * 8

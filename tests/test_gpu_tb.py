@@ -102,7 +102,7 @@ def test_tb_single_step_out_of_place(cfl, O, N, lam):
 
 def test_tb_overlap_checks(cfl):
     """Overlapping buffers are refused; adjacent ones of unequal length (hi, then
-    the 2x larger work buffer, then lo, in one allocation) are accepted."""
+    the work buffer of cf_rk4_tb_work_bytes, then lo, in one allocation) are accepted."""
     N = 16
     p = cfl.kg_params(N, 16.0, 1.0, 0.0, cfl.radiation(**FRW))
     x = torch.zeros(4 * N ** 3, dtype=torch.float64, device="cuda")
@@ -114,11 +114,42 @@ def test_tb_overlap_checks(cfl):
     y = rand_state(N, 25, amp=0.5)
     hi = y.astype(np.float32)
     lo = (y - hi.astype(np.float64)).astype(np.float32)
-    big = torch.empty(4 * n, dtype=torch.float32, device="cuda")
+    w = cfl.tb_work_bytes(N, cfl.CF_PREC_F32C) // 4  # work buffer, in floats
+    big = torch.empty(2 * n + w, dtype=torch.float32, device="cuda")
     big[:n] = dev(hi)
-    big[3 * n:] = dev(lo)
-    cfl.rk4_run_tb_f32c(p, big[:n], big[3 * n:], 1.0, 0, 1, 0.05, big[n:3 * n])
+    big[n + w:] = dev(lo)
+    with pytest.raises(cfl.CfError):  # a work buffer one float short of the documented size
+        cfl.rk4_run_tb_f32c(p, big[:n], big[n + w - 1:2 * n + w - 1], 1.0, 0, 1, 0.05, big[n:n + w])
+    cfl.rk4_run_tb_f32c(p, big[:n], big[n + w:], 1.0, 0, 1, 0.05, big[n:n + w])
     h2, l2 = dev(hi), dev(lo)
     cfl.rk4_run_f32c(p, h2, l2, 1.0, 0, 1, 0.05, cfl.alloc_work(N, cfl.CF_PREC_F32C))
     torch.cuda.synchronize()
-    assert torch.equal(big[:n], h2) and torch.equal(big[3 * n:], l2)
+    assert torch.equal(big[:n], h2) and torch.equal(big[n + w:], l2)
+
+
+@pytest.mark.parametrize("steps", [3, 4])
+def test_tb_f32c_work_placement_keeps_bits(cfl, steps):
+    """hi, lo and the work buffer carved from one allocation at offsets that are not
+    multiples of 4 KB (the run places its second hi and lo inside work at the caller's
+    residues modulo 4 KB) give the bits of separately allocated buffers; an odd step
+    count ends with the copy back from the work buffer."""
+    N = 64
+    n = 2 * N ** 3
+    print(f"case f32c work placement steps={steps}", flush=True)
+    p = cfl.kg_params(N, float(N), 1.0, 0.1, cfl.radiation(**FRW))
+    y = rand_state(N, 26, amp=0.5)
+    hi = y.astype(np.float32)
+    lo = (y - hi.astype(np.float64)).astype(np.float32)
+    wb = cfl.tb_work_bytes(N, cfl.CF_PREC_F32C)
+    o_lo, o_w = 4 * n + 256, 8 * n + 256 + 1280  # byte offsets of lo and work
+    raw = torch.empty(o_w + wb, dtype=torch.uint8, device="cuda")
+    hv = raw[:4 * n].view(torch.float32)
+    lv = raw[o_lo:o_lo + 4 * n].view(torch.float32)
+    wv = raw[o_w:o_w + wb]
+    hv.copy_(dev(hi))
+    lv.copy_(dev(lo))
+    cfl.rk4_run_tb_f32c(p, hv, lv, 1.0, 0, steps, 0.05, wv)
+    h2, l2 = dev(hi), dev(lo)
+    cfl.rk4_run_tb_f32c(p, h2, l2, 1.0, 0, steps, 0.05, cfl.alloc_tb_work(N, cfl.CF_PREC_F32C))
+    torch.cuda.synchronize()
+    assert torch.equal(hv, h2) and torch.equal(lv, l2)
