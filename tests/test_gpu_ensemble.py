@@ -41,10 +41,10 @@ def make(cfl, prec, lam=0.1):
     return ens
 
 
-def check_single_lattice_runs(cfl, ens, prec, init, t0, k0, steps, dt):
+def check_single_lattice_runs(cfl, ens, prec, init, t0, k0, steps, dt, lam=0.1):
     """Every realisation of `ens` equals the single-lattice run from its initial state."""
     for r, N in enumerate(ens.Ns):
-        p = cfl.kg_params(N, float(N), 1.0, 0.1, cfl.radiation(**FRW))
+        p = cfl.kg_params(N, float(N), 1.0, lam, cfl.radiation(**FRW))
         if prec == cfl.CF_PREC_F32C:
             hi, lo = init[r]
             cfl.rk4_run_f32c(p, hi, lo, t0, k0, steps, dt, cfl.alloc_work(N, prec))
@@ -91,6 +91,21 @@ def test_region_shapes_keep_bits(cfl, prec, grid):
     init = [raw_state(cfl, ens, r) for r in range(ens.n)]
     ens.run(t0, k0, steps, dt)
     check_single_lattice_runs(cfl, ens, prec, init, t0, k0, steps, dt)
+    ens.close()
+
+
+@pytest.mark.parametrize("prec,grid", [(0, 0), (1, 0), (0, -1), (1, -1)])
+def test_ensemble_without_lambda(cfl, prec, grid):
+    """lambda = 0 selects the kernels without the phi^3 term; both region shapes."""
+    print(f"case ensemble lambda=0 prec={prec} grid={grid}", flush=True)
+    ens = make(cfl, prec, lam=0.0)
+    if grid:
+        ens.set_plan(0, 0, grid)
+    ens.set_region(40, cfl.CF_ENS_REGION_24X40)
+    t0, k0, steps, dt = 1.0, 0, 2, 0.05
+    init = [raw_state(cfl, ens, r) for r in range(ens.n)]
+    ens.run(t0, k0, steps, dt)
+    check_single_lattice_runs(cfl, ens, prec, init, t0, k0, steps, dt, lam=0.0)
     ens.close()
 
 
