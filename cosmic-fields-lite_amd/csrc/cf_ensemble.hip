@@ -579,14 +579,16 @@ cf_status cf_ensemble_tune(cf_ensemble *e, double dt, int reps, int *grid_out) {
   e->grid = best_g;
   CF_TRY(ens_build(e));
   // per lattice size, every (region shape, chunk) pair, the other sizes' plans held;
-  // a second sweep revisits each size once the others have their tuned plans
+  // a second sweep revisits each size once the others have their tuned plans. Chunks
+  // go longest first and a candidate must win by more than 0.5%: within the timing
+  // noise the longer chunk (fewer warm-up planes) is kept.
   for (int sweep = 0; sweep < 2; ++sweep) {
     for (auto &kv : e->chunk) {
       const int64_t N = kv.first;
       int best_a = kv.second, best_r = e->region[N];
       CF_TRY(ens_time(e, reps, dt, &best));
       std::vector<int> chunks;
-      for (int A : {8, 16, 32, 64, 128, 256, (int)N})
+      for (int A : {(int)N, 256, 128, 64, 32, 16, 8})
         if (A <= N && std::find(chunks.begin(), chunks.end(), A) == chunks.end()) chunks.push_back(A);
       for (int reg : {CF_ENS_REGION_32X32, CF_ENS_REGION_24X40})
         for (int A : chunks) {
@@ -595,7 +597,7 @@ cf_status cf_ensemble_tune(cf_ensemble *e, double dt, int reps, int *grid_out) {
           e->region[N] = reg;
           CF_TRY(ens_build(e));
           CF_TRY(ens_time(e, reps, dt, &ms));
-          if (ms < best) {
+          if (ms < best * 0.995f) {
             best = ms;
             best_a = A;
             best_r = reg;
