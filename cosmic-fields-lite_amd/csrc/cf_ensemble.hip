@@ -1,12 +1,14 @@
 // cf_ensemble.hip -- ensembles of lattices of uneven N advanced together.
 //
-// All realisations of an ensemble take their RK4 steps in one persistent
-// kernel per step: a fixed grid of 1024-thread blocks walks a table of work
-// units (one output tile of one lattice over a chunk of a-planes, the unit of
-// the temporally blocked step, cf_tb_body.hpp). Units are assigned to blocks on
-// the host by longest-processing-time-first over their iteration counts, so
-// the mix of N = 64 / 96 / 128 lattices finishes together; each block then runs
-// its list in order, with no atomics and no inter-block communication.
+// All realisations of an ensemble take their RK4 steps in one kernel per step
+// over a table of work units (one output tile of one lattice over a chunk of
+// a-planes, the unit of the temporally blocked step, cf_tb_body.hpp). Two launch
+// forms: a fixed grid of 1024-thread blocks, each running a list of units that
+// the host balanced by longest-processing-time-first over their iteration counts
+// (so the mix of N = 64 / 96 / 128 lattices finishes together), or one block per
+// unit, longest first. No atomics and no inter-block communication. The block's
+// region shape (32 x 32 or 24 x 40 columns, so 24 x 24 or 16 x 32 tiles) and the
+// chunk are chosen per lattice size; the autotuner times the candidates.
 //
 // The step scalars a(t), H(t) of every stage live in a device table, so the
 // kernels of a whole run can be captured once into a hipGraph and replayed
