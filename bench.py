@@ -7,12 +7,14 @@ background, N=512^3 lattice, L=512, m=1, a(t) = (t/t_i)^(1/2) with t_i = 1
 exp(-k^2/(2*0.25^2)), rms 1), dphi/dt = 0, classic RK4 with dt = 0.05.
 
 A "step" is one RK4 step over the whole lattice. Run A stores the fields in
-FP32 with a compensated hi+lo state and does all arithmetic in double; run B is
-full FP64. Each is timed with two kernel schedules on the same initial state:
-one temporally blocked kernel per step (tb_*) and four fused stage kernels per
-step; all four produce the bits of the reference's arithmetic order (FP64) or
-of the compensated scheme (FP32-comp). The headline `value` is run A with its
-faster schedule. Inputs are resident in HBM before the timed region, and the GPU
+FP32 with a compensated hi+lo state, in two arithmetic forms: FP32-comp (f32c, all
+arithmetic in double) and the single-precision stencil (f32s, float forces on the
+float stage inputs, the increment and the state update in double); run B is full
+FP64. They are timed with one temporally blocked kernel per step (tb_*) and, for f32c
+and FP64, with four fused stage kernels per step; every path produces the bits of
+its scheme (FP64: the reference's arithmetic order). Before the f32s path is timed,
+the library's tuner picks the pace of its lock-step for this lattice (untimed set-up,
+about 0.4 s at N=512). The headline `value` is run A's fastest path. Inputs are resident in HBM before the timed region, and the GPU
 has run 0.3 s of untimed steps before the first path, so that every path is timed at
 settled clocks (each then does its own --warmup steps). The line
 also carries the roofline of the headline kernel (HIP events on the launch
@@ -46,18 +48,20 @@ HBM_PEAK_GBS = 8000.0
 # The FP64 kernels here keep the partial RK4 sum instead of D, so s4 does not
 # read x and they move 16 passes (256 B); the canonical 272 B is still the
 # figure used for `achieved`.
-BYTES_PER_SITE_STEP = {"f64": 272, "f32c": 152}
-DESIGN_BYTES_PER_SITE_STEP = {"f64": 256, "f32c": 152, "tb_f64": 32, "tb_f32c": 32}
+BYTES_PER_SITE_STEP = {"f64": 272, "f32c": 152, "f32s": 152}
+DESIGN_BYTES_PER_SITE_STEP = {"f64": 256, "f32c": 152, "tb_f64": 32, "tb_f32c": 32, "tb_f32s": 32}
 KERNEL_NAMES = {"f64": "cf::k_march<double, 1, S, false> (4 stage kernels)",
                 "f32c": "cf::k_march<float, 2, S, false> (4 stage kernels)",
                 "tb_f64": "cf::k_rk4_tb_ring<false>",
-                "tb_f32c": "cf::k_rk4_tb<float, true, false>"}
+                "tb_f32c": "cf::k_rk4_tb<float, true, false>",
+                "tb_f32s": "cf::k_rk4_tb_sa<false, true>"}  # paced; <false, false> when the tuner picks 0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "rk4_tb_N512_pmc_traffic.md")
 # What bounds each path's kernel at N=512, from the committed PMC summaries: the FP32-comp
 # blocked kernel issues VALU 76% of its cycles (float <-> double conversions on top of the
 # FP64 arithmetic); the FP64 blocked kernel runs as fast with its arithmetic cut to a
 # skeleton, so its time is the memory traffic including the halo re-reads.
 LIMITER = {"tb_f32c": "vector-ALU issue", "tb_f64": "memory: HBM + L2 misses on halo re-reads",
+           "tb_f32s": "memory: halo-inclusive plane loads at the paced lock-step (L2 + HBM)",
            "f32c": "hbm", "f64": "hbm"}
 
 
@@ -122,9 +126,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--chunk", type=int, default=0, help="a-planes per block of the temporally blocked kernel")
-    ap.add_argument("--paths", default="tb_f32c,tb_f64,f32c,f64",
-                    help="headline: kernels to time (tb_f32c, tb_f64: one temporally blocked kernel per step; "
-                         "f32c, f64: four fused stage kernels per step)")
+    ap.add_argument("--paths", default="tb_f32s,tb_f32c,tb_f64,f32c,f64",
+                    help="headline: kernels to time (tb_f32s, tb_f32c, tb_f64: one temporally blocked kernel "
+                         "per step; f32c, f64: four fused stage kernels per step)")
+    ap.add_argument("--run-a", choices=("f32s", "f32c"), default="f32s",
+                    help="precision: the arithmetic of run A (f32s: float stencil, double increment; "
+                         "f32c: FP64)")
     ap.add_argument("--save-dir", default="", help="ensemble: where to save the averaged density spectra (.npy)")
     ap.add_argument("--workload", choices=("headline", "ensemble", "precision", "adjoint", "variants", "slab"),
                     default="headline",
@@ -287,8 +294,8 @@ def observer_timing(cfl, plan, x, N, L, m, cosmo, t, stream, reps=5):
 
 
 def precision_workload(args):
-    """Run A (FP32 hi + FP32 compensation, FP64 arithmetic) and run B (FP64) of the
-    N=512 FRW workload from the same initial state (seed 2, P(k) ~ exp(-k^2/(2*0.25^2)),
+    """Run A (FP32 hi + FP32 compensation; --run-a f32s: float stencil arithmetic with a
+    double increment, f32c: FP64 arithmetic) and run B (FP64) of the N=512 FRW workload from the same initial state (seed 2, P(k) ~ exp(-k^2/(2*0.25^2)),
     rms 1, dt 0.05) for --steps steps (1000 for the full study). Every
     --report-every steps: relative L2 error of phi and dphi/dt of A against B, and
     the largest relative error over non-empty bins of the field (mode) spectrum
@@ -304,6 +311,7 @@ def precision_workload(args):
     hi, lo = cfl.split_f64(xB)
     wA = cfl.alloc_tb_work(N, cfl.CF_PREC_F32C)
     wB = cfl.alloc_tb_work(N, cfl.CF_PREC_F64)
+    run_a = cfl.rk4_run_tb_f32s if args.run_a == "f32s" else cfl.rk4_run_tb_f32c
     rho = torch.empty(N ** 3, dtype=torch.float64, device="cuda")
     n3 = N ** 3
 
@@ -329,7 +337,7 @@ def precision_workload(args):
     w0 = time.perf_counter()
     while done < args.steps:
         k = min(args.report_every, args.steps - done)
-        cfl.rk4_run_tb_f32c(p, hi, lo, t0, done, k, dt, wA)
+        run_a(p, hi, lo, t0, done, k, dt, wA)
         cfl.rk4_run_tb_f64(p, xB, t0, done, k, dt, wB)
         done += k
         t = t0 + done * dt
@@ -348,11 +356,13 @@ def precision_workload(args):
     last = rows[-1]
     worst = max(last["field_spectrum_max_rel_6dec"], last["rho_spectrum_max_rel_6dec"])
     line = {
-        "metric": "relative error of FP32-comp (run A) vs FP64 (run B), N=512^3 FRW",
+        "metric": f"relative error of FP32-comp run A ({args.run_a}) vs FP64 (run B), N=512^3 FRW",
         "value": worst,
         "unit": "largest per-bin relative spectrum error (bins within 6 decades of the peak), last step",
         "higher_is_better": False,
-        "n_gpus": 1, "steps": args.steps, "dtype": "f64", "storage": "A: f32 hi + f32 compensation; B: f64",
+        "n_gpus": 1, "steps": args.steps,
+        "dtype": "A: f32 stencil, f64 increment; B: f64" if args.run_a == "f32s" else "f64",
+        "storage": "A: f32 hi + f32 compensation; B: f64",
         "data": "synthetic (Gaussian random field, seed 2)",
         "config": {"workload": f"KG-FRW radiation era N={N}^3 L={L:g} m=1 dt=0.05 RK4, A vs B every "
                                f"{args.report_every} steps", "wall_s": time.perf_counter() - w0},
@@ -591,8 +601,17 @@ def main():
             else:
                 bufs = [cfl.split_f64(x0), tuple(torch.empty(2 * N ** 3, dtype=torch.float32, device="cuda")
                                                  for _ in range(2))]
-                step = lambda k, i, o: cfl.rk4_step_tb_f32c(p, i[0], i[1], o[0], o[1], t0 + k * dt, dt,  # noqa: E731
-                                                            args.chunk)
+                f = cfl.rk4_step_tb_f32c if mode == "tb_f32c" else cfl.rk4_step_tb_f32s
+                step = (lambda f: lambda k, i, o: f(p, i[0], i[1], o[0], o[1], t0 + k * dt, dt,  # noqa: E731
+                                                    args.chunk))(f)
+                if mode == "tb_f32s":
+                    # untimed set-up: the pace of the lock-step for this lattice (the state is
+                    # left as it was; the tuner uses its own output buffers)
+                    tw = cfl.alloc_tb_work(N, cfl.CF_PREC_F32S)
+                    t_tune = time.perf_counter()
+                    pace = cfl.rk4_tb_tune_pace(p, bufs[0][0], bufs[0][1], t0, dt, tw, args.chunk)
+                    tune_s = time.perf_counter() - t_tune
+                    del tw
 
             def run(k0, n, bufs=bufs, step=step):
                 for k in range(k0, k0 + n):
@@ -619,19 +638,25 @@ def main():
         rate = N ** 3 / (ms * 1e-3) / 1e9
         canon = BYTES_PER_SITE_STEP[mode.replace("tb_", "")]
         kb = DESIGN_BYTES_PER_SITE_STEP[mode]
+        kname = KERNEL_NAMES[mode]
+        if mode == "tb_f32s" and pace == 0:
+            kname = "cf::k_rk4_tb_sa<false, false>"
         results[mode] = {"value": rate, "ms_per_step": ms, "wall_s": wall,
-                         "kernel": KERNEL_NAMES[mode],
+                         "kernel": kname,
                          "kernel_bytes_per_site_step": kb,
                          "kernel_frac_of_8TBs": kb * rate / HBM_PEAK_GBS,
                          "canonical_bytes_per_site_step": canon,
                          "canonical_frac_of_8TBs": canon * rate / HBM_PEAK_GBS}
+        if mode == "tb_f32s":
+            results[mode]["pace_ticks_per_plane"] = pace
+            results[mode]["pace_tune_s"] = tune_s
         print(json.dumps({"path": mode, **results[mode]}), file=sys.stderr, flush=True)
         run = step = bufs = work = None
         torch.cuda.synchronize()
 
     # Headline: run A (FP32-compensated storage), the fastest of its kernels; the
     # FP64 paths (run B) are reported beside it.
-    a_modes = [q for q in ("tb_f32c", "f32c") if q in results]
+    a_modes = [q for q in ("tb_f32s", "tb_f32c", "f32c") if q in results]
     b_modes = [q for q in ("tb_f64", "f64") if q in results]
     best_a = max(a_modes, key=lambda q: results[q]["value"]) if a_modes else None
     best_b = max(b_modes, key=lambda q: results[q]["value"]) if b_modes else None
@@ -656,8 +681,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "storage": "f32 hi + f32 compensation" if "f32c" in head_mode else "f64",
+        "dtype": "f32 stencil, f64 increment" if head_mode == "tb_f32s" else "f64",
+        "storage": "f32 hi + f32 compensation" if ("f32c" in head_mode or "f32s" in head_mode) else "f64",
         "data": "synthetic (Gaussian random field, seed 2)",
         "config": {"workload": f"KG-FRW radiation era N={N}^3 L={L:g} m=1 dt=0.05 RK4 "
                                f"(run A FP32-comp, run B FP64)", "N": N, "L": L, "dt": dt,
@@ -684,8 +709,8 @@ def main():
                              "(CUs x kernel cycles) from profiles/"},
     }
     if best_a and best_b:
-        line["speedup_f32c_vs_fastest_f64"] = results[best_a]["value"] / results[best_b]["value"]
-    for a, b in (("tb_f32c", "tb_f64"), ("f32c", "f64")):
+        line["speedup_run_a_vs_fastest_f64"] = results[best_a]["value"] / results[best_b]["value"]
+    for a, b in (("tb_f32s", "tb_f64"), ("tb_f32c", "tb_f64"), ("f32c", "f64")):
         if a in results and b in results:
             line[f"speedup_{a}_vs_{b}"] = results[a]["value"] / results[b]["value"]
     if not args.no_cpu:

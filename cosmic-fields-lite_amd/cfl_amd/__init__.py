@@ -12,7 +12,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import (CF_ENS_GRID_FLAT, CF_ENS_REGION_24X40, CF_ENS_REGION_32X32, CF_PREC_F32C, CF_PREC_F64, CF_SPEC_DERIV_POWER_CUTOFF, CF_SPEC_GAUSS,  # noqa: F401
+from ._lib import (CF_ENS_GRID_FLAT, CF_ENS_REGION_24X40, CF_ENS_REGION_32X32, CF_PREC_F32C, CF_PREC_F32S, CF_PREC_F64, CF_SPEC_DERIV_POWER_CUTOFF, CF_SPEC_GAUSS,  # noqa: F401
                    CF_SPEC_KM3_GAUSS, CF_SPEC_POWER_CUTOFF, CF_SPEC_SCALE_INV,
                    CF_VAR_APPROX_COMOVING_CURVATURE, CF_VAR_COMOVING_CURVATURE, CF_VAR_FIXED_CURVATURE,
                    CF_VAR_SQRT_POTENTIAL, CfError, Cosmology, KGParams, check)
@@ -153,6 +153,30 @@ def rk4_run_tb_f32c(p, hi, lo, t0, k0, n_steps, dt, work, chunk=0, stream=None):
                                  _stream(stream)))
 
 
+def rk4_run_tb_f32s(p, hi, lo, t0, k0, n_steps, dt, work, chunk=0, stream=None):
+    """Compensated storage with the single-precision stencil (float forces, double
+    increment); work from alloc_tb_work(N, CF_PREC_F32S)."""
+    check(lib.cf_rk4_run_tb_f32s(C.byref(p), _ptr(hi), _ptr(lo), t0, k0, n_steps, dt, chunk, _ptr(work),
+                                 _stream(stream)))
+
+
+def rk4_tb_tune_pace(p, hi, lo, t, dt, work, chunk=0, stream=None):
+    """Time the single-precision-stencil step of this lattice at a range of paces and
+    keep the best for its later runs and steps (state unchanged); returns the pace."""
+    out = C.c_int(0)
+    check(lib.cf_rk4_tb_tune_pace(C.byref(p), _ptr(hi), _ptr(lo), t, dt, chunk, _ptr(work), _stream(stream),
+                                  C.byref(out)))
+    return out.value
+
+
+def rk4_tb_set_pace(N, pace, chunk=0):
+    check(lib.cf_rk4_tb_set_pace(N, chunk, pace))
+
+
+def rk4_tb_get_pace(N, chunk=0):
+    return lib.cf_rk4_tb_get_pace(N, chunk)
+
+
 def rk4_step_tb_f64(p, x_in, x_out, t, dt, chunk=0, stream=None):
     """One temporally blocked step out of place: x_out = step(x_in)."""
     check(lib.cf_rk4_step_tb_f64(C.byref(p), _ptr(x_in), _ptr(x_out), t, dt, chunk, _stream(stream)))
@@ -174,6 +198,11 @@ def rk4_step_tb_f32c_slab(p, n_planes, hi_in, lo_in, hi_out, lo_out, t, dt, chun
 
 def rk4_step_tb_f32c(p, hi_in, lo_in, hi_out, lo_out, t, dt, chunk=0, stream=None):
     check(lib.cf_rk4_step_tb_f32c(C.byref(p), _ptr(hi_in), _ptr(lo_in), _ptr(hi_out), _ptr(lo_out), t, dt, chunk,
+                                  _stream(stream)))
+
+
+def rk4_step_tb_f32s(p, hi_in, lo_in, hi_out, lo_out, t, dt, chunk=0, stream=None):
+    check(lib.cf_rk4_step_tb_f32s(C.byref(p), _ptr(hi_in), _ptr(lo_in), _ptr(hi_out), _ptr(lo_out), t, dt, chunk,
                                   _stream(stream)))
 
 

@@ -20,6 +20,7 @@ CF_E_UNSUPPORTED = 5
 
 CF_PREC_F64 = 0
 CF_PREC_F32C = 1
+CF_PREC_F32S = 2
 
 CF_NOALLOC_MAX_N = 128  # largest N of cf_rk4_run_tb_f64_noalloc
 
@@ -97,6 +98,11 @@ _SIGS = [
     ("cf_rk4_step_tb_f64_slab", st, [C.POINTER(KGParams), i64, vp, vp, dbl, dbl, C.c_int, vp]),
     ("cf_rk4_step_tb_f32c_slab", st, [C.POINTER(KGParams), i64, vp, vp, vp, vp, dbl, dbl, C.c_int, vp]),
     ("cf_rk4_run_tb_f32c", st, [C.POINTER(KGParams), vp, vp, dbl, i64, i64, dbl, C.c_int, vp, vp]),
+    ("cf_rk4_step_tb_f32s", st, [C.POINTER(KGParams), vp, vp, vp, vp, dbl, dbl, C.c_int, vp]),
+    ("cf_rk4_run_tb_f32s", st, [C.POINTER(KGParams), vp, vp, dbl, i64, i64, dbl, C.c_int, vp, vp]),
+    ("cf_rk4_tb_tune_pace", st, [C.POINTER(KGParams), vp, vp, dbl, dbl, C.c_int, vp, vp, C.POINTER(C.c_int)]),
+    ("cf_rk4_tb_set_pace", st, [i64, C.c_int, C.c_int]),
+    ("cf_rk4_tb_get_pace", C.c_int, [i64, C.c_int]),
     ("cf_ensemble_create", st, [C.c_int, vp, vp, dbl, dbl, C.POINTER(Cosmology), C.c_int, C.POINTER(vp)]),
     ("cf_ensemble_destroy", st, [vp]),
     ("cf_ensemble_sites", i64, [vp]),

@@ -118,6 +118,7 @@ __device__ __forceinline__ void ens_unit(TbLds<T> &lds, const EnsReal *__restric
   u.T = e.T;
   u.Tc = e.Tc;
   u.NA = N;
+  u.pace = 0;
   tb_unit<T, F32C, LAM, DEFER, LANE_NBR, TB_KG, false, RB, RC>(lds, u, k, xin, lin, xout, lout);
 }
 

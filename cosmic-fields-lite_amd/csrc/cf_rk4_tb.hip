@@ -9,7 +9,9 @@
 // state), so runs ping-pong between two state arrays.
 #include <cmath>
 
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "cf_tb_body.hpp"
@@ -27,6 +29,7 @@ struct TbGeom {
   int total;         // ntb * ntc * n_ac
   int64_t NA;        // planes of the state buffers along a (N, or a slab's planes + ghosts)
   int64_t a_lo, a_hi;  // output planes [a_lo, a_hi)
+  int pace;            // ticks of the 100 MHz real-time counter per plane iteration, 0: unpaced
 };
 
 // Work unit of logical block `bid`: tile (tb, tc), chunk ac, c fastest.
@@ -44,6 +47,7 @@ __device__ __forceinline__ TbUnit tb_unit_of(const TbGeom &g, int bid) {
   u.a0 = g.a_lo + (int64_t)ac * g.A;
   u.a1 = (u.a0 + g.A < g.a_hi) ? u.a0 + g.A : g.a_hi;
   u.NA = g.NA;
+  u.pace = g.pace;
   return u;
 }
 
@@ -64,6 +68,22 @@ __global__ __launch_bounds__(1024) void k_rk4_tb(TbGeom g, TbStep k, const T *__
   // float plane buffers are cheap and whose VALU load is the larger (measured).
   constexpr int RB = F32C ? TB_F32C_RB : TBR, RC = F32C ? TB_F32C_RC : TBR;
   tb_unit<T, F32C, LAM, true, !F32C && RC == TBR, TB_KG, false, RB, RC>(lds, u, k, xin, lin, xout, lout);
+}
+
+// The compensated storage with the single-precision stencil (float forces, double
+// increment): the same body and geometry as the FP32-comp kernel; PACE: paced
+// (cf_tb_body.hpp), at g.pace ticks per plane iteration. Occupancy stays at one
+// 1024-thread block per CU, so the compiler may use the 128 VGPRs that allows: with
+// its default target it kept 84 and serialised the LDS neighbour reads (read, wait,
+// add, read), 77.6 against 80.4 GSite*step/s unpaced at N=512.
+template <bool LAM, bool PACE>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_rk4_tb_sa(
+    TbGeom g, TbStep k, TbStepF kf, const float *__restrict__ xin, const float *__restrict__ lin,
+    float *__restrict__ xout, float *__restrict__ lout) {
+  __shared__ TbLds<float> lds;
+  const TbUnit u = tb_unit_of(g, tb_xcd_order(blockIdx.x, g.total));
+  tb_unit<float, true, LAM, true, false, TB_KG, false, TB_F32C_RB, TB_F32C_RC, true, PACE>(lds, u, k, xin, lin, xout,
+                                                                                       lout, nullptr, TbPsi{}, kf);
 }
 
 // FP64: the same step with the state streamed into an LDS ring TB_RING_DEPTH planes
@@ -106,6 +126,7 @@ static TbGeom tb_geom(int64_t N, int A, int RB = TBR, int RC = TBR, int64_t a_lo
   g.A = (A <= 0 || A > planes) ? (int)planes : A;
   g.n_ac = (int)cdiv(planes, g.A);
   g.total = g.ntb * g.ntc * g.n_ac;
+  g.pace = 0;
   return g;
 }
 
@@ -178,6 +199,37 @@ static cf_status tb_launch(const TbGeom &g, const TbStep &k, const T *xin, const
   return CF_OK;
 }
 
+// The single-precision stencil's scalars: TbStep's RHS coefficients and stage-input
+// steps rounded to float.
+static TbStepF tb_step_f32(const TbStep &k) {
+  TbStepF f;
+  for (int s = 0; s < 4; ++s) {
+    f.c_h[s] = (float)k.c_h[s];
+    f.inv[s] = (float)k.inv[s];
+  }
+  f.c_m = (float)k.c_m;
+  f.lam = (float)k.lam;
+  f.dh = (float)k.dh;
+  f.dt = (float)k.dt;
+  return f;
+}
+
+static cf_status tb_launch_sa(const TbGeom &g, const TbStep &k, const float *xin, const float *lin, float *xout,
+                              float *lout, hipStream_t st) {
+  const TbStepF kf = tb_step_f32(k);
+  const bool lam = k.lam != 0.0, paced = g.pace > 0;
+  if (lam && paced)
+    hipLaunchKernelGGL((k_rk4_tb_sa<true, true>), dim3(g.total), dim3(1024), 0, st, g, k, kf, xin, lin, xout, lout);
+  else if (lam)
+    hipLaunchKernelGGL((k_rk4_tb_sa<true, false>), dim3(g.total), dim3(1024), 0, st, g, k, kf, xin, lin, xout, lout);
+  else if (paced)
+    hipLaunchKernelGGL((k_rk4_tb_sa<false, true>), dim3(g.total), dim3(1024), 0, st, g, k, kf, xin, lin, xout, lout);
+  else
+    hipLaunchKernelGGL((k_rk4_tb_sa<false, false>), dim3(g.total), dim3(1024), 0, st, g, k, kf, xin, lin, xout, lout);
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
 // [a, a + na) and [b, b + nb) do not overlap.
 static bool disjoint(const void *a, size_t na, const void *b, size_t nb) {
   const char *x = static_cast<const char *>(a), *y = static_cast<const char *>(b);
@@ -219,6 +271,80 @@ cf_status tb_step_variant(int var, const cf_kg_params *p, double f_a, const doub
   return CF_OK;
 }
 
+// Pacing of the single-precision-stencil kernel per device, lattice size and chunk (ticks
+// of the 100 MHz real-time counter per plane iteration; absent or 0: unpaced), set by
+// cf_rk4_tb_tune_pace or cf_rk4_tb_set_pace.
+static std::mutex g_pace_mu;
+static std::map<std::tuple<int, int64_t, int>, int> g_pace;
+
+static cf_status tb_pace_key(int64_t N, int A, std::tuple<int, int64_t, int> *key) {
+  int dev = 0;
+  CF_HIP(hipGetDevice(&dev));
+  *key = std::make_tuple(dev, N, A);
+  return CF_OK;
+}
+static cf_status tb_pace_lookup(TbGeom &g) {
+  std::tuple<int, int64_t, int> key;
+  CF_TRY(tb_pace_key(g.N, g.A, &key));
+  std::lock_guard<std::mutex> lock(g_pace_mu);
+  const auto it = g_pace.find(key);
+  g.pace = it == g_pace.end() ? 0 : it->second;
+  return CF_OK;
+}
+static cf_status tb_pace_store(int64_t N, int A, int pace) {
+  std::tuple<int, int64_t, int> key;
+  CF_TRY(tb_pace_key(N, A, &key));
+  std::lock_guard<std::mutex> lock(g_pace_mu);
+  g_pace[key] = pace;
+  return CF_OK;
+}
+
+// Runs and single steps of the compensated storage: FP64 arithmetic (sa = false) or
+// the single-precision stencil (sa = true).
+static cf_status tb_launch_comp(bool sa, const TbGeom &g, const TbStep &k, const float *xin, const float *lin,
+                                float *xout, float *lout, hipStream_t st) {
+  return sa ? tb_launch_sa(g, k, xin, lin, xout, lout, st) : tb_launch<float, true>(g, k, xin, lin, xout, lout, st);
+}
+
+static cf_status tb_run_comp(const char *name, bool sa, const cf_kg_params *p, float *hi, float *lo, double t0,
+                             int64_t k0, int64_t n_steps, double dt, int chunk, void *work, hipStream_t st) {
+  CF_TRY(tb_check(p));
+  const size_t n = 2 * (size_t)p->N * p->N * p->N;
+  const size_t wb = 2 * n * sizeof(float) + 2 * TB_ALIGN_SLACK;  // cf_rk4_tb_work_bytes
+  CF_REQUIRE(hi && lo && work && disjoint(hi, lo, n * sizeof(float)) &&
+                 disjoint(hi, n * sizeof(float), work, wb) && disjoint(lo, n * sizeof(float), work, wb),
+             "%s: null or overlapping buffer", name);
+  CF_REQUIRE(n_steps >= 0, "%s: negative step count", name);
+  TbGeom g = tb_f32c_geom(p->N, chunk);
+  if (sa) CF_TRY(tb_pace_lookup(g));
+  float *his[2] = {hi, same_residue(work, hi)};
+  float *los[2] = {lo, same_residue(his[1] + n, lo)};
+  for (int64_t s = 0; s < n_steps; ++s) {
+    const double t = t0 + static_cast<double>(k0 + s) * dt;
+    CF_TRY(tb_launch_comp(sa, g, tb_step_host(p, t, dt), his[s & 1], los[s & 1], his[(s + 1) & 1],
+                          los[(s + 1) & 1], st));
+  }
+  if (n_steps & 1) {
+    CF_HIP(hipMemcpyAsync(hi, his[1], n * sizeof(float), hipMemcpyDeviceToDevice, st));
+    CF_HIP(hipMemcpyAsync(lo, los[1], n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  }
+  return CF_OK;
+}
+
+static cf_status tb_step_comp(const char *name, bool sa, const cf_kg_params *p, const float *hi_in,
+                              const float *lo_in, float *hi_out, float *lo_out, double t, double dt, int chunk,
+                              hipStream_t st) {
+  CF_TRY(tb_check(p));
+  const size_t b = 2 * (size_t)p->N * p->N * p->N * sizeof(float);
+  CF_REQUIRE(hi_in && lo_in && hi_out && lo_out, "%s: null buffer", name);
+  CF_REQUIRE(disjoint(hi_in, lo_in, b) && disjoint(hi_out, lo_out, b) && disjoint(hi_in, hi_out, b) &&
+                 disjoint(hi_in, lo_out, b) && disjoint(lo_in, hi_out, b) && disjoint(lo_in, lo_out, b),
+             "%s: overlapping buffers", name);
+  TbGeom g = tb_f32c_geom(p->N, chunk);
+  if (sa) CF_TRY(tb_pace_lookup(g));
+  return tb_launch_comp(sa, g, tb_step_host(p, t, dt), hi_in, lo_in, hi_out, lo_out, st);
+}
+
 // Ping-pong buffer of the entry point that allocates nothing
 // (cf_rk4_run_tb_f64_noalloc): 2 N^3 doubles for N <= CF_NOALLOC_MAX_N, one copy per
 // GPU, held until the process ends. In host code `scratch` names a host-side
@@ -254,7 +380,7 @@ extern "C" {
 size_t cf_rk4_tb_work_bytes(int64_t N, int precision) {
   if (!valid_N(N)) return 0;
   const size_t n = 2 * (size_t)N * N * N;
-  return precision == CF_PREC_F32C ? 2 * n * sizeof(float) + 2 * TB_ALIGN_SLACK : n * sizeof(double);
+  return (precision == CF_PREC_F32C || precision == CF_PREC_F32S) ? 2 * n * sizeof(float) + 2 * TB_ALIGN_SLACK : n * sizeof(double);
 }
 
 cf_status cf_rk4_run_tb_f64(const cf_kg_params *p, double *x, double t0, int64_t k0, int64_t n_steps, double dt,
@@ -302,26 +428,12 @@ cf_status cf_rk4_run_tb_f64_noalloc(const cf_kg_params *p, double *x, double t0,
 
 cf_status cf_rk4_run_tb_f32c(const cf_kg_params *p, float *hi, float *lo, double t0, int64_t k0, int64_t n_steps,
                              double dt, int chunk, void *work, cf_stream stream) {
-  CF_TRY(tb_check(p));
-  const size_t n = 2 * (size_t)p->N * p->N * p->N;
-  const size_t wb = 2 * n * sizeof(float) + 2 * TB_ALIGN_SLACK;  // cf_rk4_tb_work_bytes
-  CF_REQUIRE(hi && lo && work && disjoint(hi, lo, n * sizeof(float)) &&
-                 disjoint(hi, n * sizeof(float), work, wb) && disjoint(lo, n * sizeof(float), work, wb),
-             "cf_rk4_run_tb_f32c: null or overlapping buffer");
-  CF_REQUIRE(n_steps >= 0, "cf_rk4_run_tb_f32c: negative step count");
-  const TbGeom g = tb_f32c_geom(p->N, chunk);
-  float *his[2] = {hi, same_residue(work, hi)};
-  float *los[2] = {lo, same_residue(his[1] + n, lo)};
-  for (int64_t s = 0; s < n_steps; ++s) {
-    const double t = t0 + static_cast<double>(k0 + s) * dt;
-    CF_TRY((tb_launch<float, true>(g, tb_step_host(p, t, dt), his[s & 1], los[s & 1], his[(s + 1) & 1],
-                                    los[(s + 1) & 1], S(stream))));
-  }
-  if (n_steps & 1) {
-    CF_HIP(hipMemcpyAsync(hi, his[1], n * sizeof(float), hipMemcpyDeviceToDevice, S(stream)));
-    CF_HIP(hipMemcpyAsync(lo, los[1], n * sizeof(float), hipMemcpyDeviceToDevice, S(stream)));
-  }
-  return CF_OK;
+  return tb_run_comp("cf_rk4_run_tb_f32c", false, p, hi, lo, t0, k0, n_steps, dt, chunk, work, S(stream));
+}
+
+cf_status cf_rk4_run_tb_f32s(const cf_kg_params *p, float *hi, float *lo, double t0, int64_t k0, int64_t n_steps,
+                             double dt, int chunk, void *work, cf_stream stream) {
+  return tb_run_comp("cf_rk4_run_tb_f32s", true, p, hi, lo, t0, k0, n_steps, dt, chunk, work, S(stream));
 }
 
 cf_status cf_rk4_step_tb_f64(const cf_kg_params *p, const double *x_in, double *x_out, double t, double dt,
@@ -367,14 +479,82 @@ cf_status cf_rk4_step_tb_f32c_slab(const cf_kg_params *p, int64_t n_planes, cons
 
 cf_status cf_rk4_step_tb_f32c(const cf_kg_params *p, const float *hi_in, const float *lo_in, float *hi_out,
                               float *lo_out, double t, double dt, int chunk, cf_stream stream) {
+  return tb_step_comp("cf_rk4_step_tb_f32c", false, p, hi_in, lo_in, hi_out, lo_out, t, dt, chunk, S(stream));
+}
+
+cf_status cf_rk4_step_tb_f32s(const cf_kg_params *p, const float *hi_in, const float *lo_in, float *hi_out,
+                              float *lo_out, double t, double dt, int chunk, cf_stream stream) {
+  return tb_step_comp("cf_rk4_step_tb_f32s", true, p, hi_in, lo_in, hi_out, lo_out, t, dt, chunk, S(stream));
+}
+
+// Pacing of the single-precision-stencil step: time the step of this lattice at a
+// range of paces and keep the fastest (the largest pace within 1% of the fastest: the
+// rate falls off a cliff just below the best pace and only slowly above it).
+cf_status cf_rk4_tb_tune_pace(const cf_kg_params *p, const float *hi, const float *lo, double t, double dt, int chunk,
+                              void *work, cf_stream stream, int *pace_out) {
   CF_TRY(tb_check(p));
-  const size_t b = 2 * (size_t)p->N * p->N * p->N * sizeof(float);
-  CF_REQUIRE(hi_in && lo_in && hi_out && lo_out, "cf_rk4_step_tb_f32c: null buffer");
-  CF_REQUIRE(disjoint(hi_in, lo_in, b) && disjoint(hi_out, lo_out, b) && disjoint(hi_in, hi_out, b) &&
-                 disjoint(hi_in, lo_out, b) && disjoint(lo_in, hi_out, b) && disjoint(lo_in, lo_out, b),
-             "cf_rk4_step_tb_f32c: overlapping buffers");
-  const TbGeom g = tb_f32c_geom(p->N, chunk);
-  return tb_launch<float, true>(g, tb_step_host(p, t, dt), hi_in, lo_in, hi_out, lo_out, S(stream));
+  const size_t n = 2 * (size_t)p->N * p->N * p->N;
+  const size_t wb = 2 * n * sizeof(float) + 2 * TB_ALIGN_SLACK;  // cf_rk4_tb_work_bytes
+  CF_REQUIRE(hi && lo && work && disjoint(hi, lo, n * sizeof(float)) &&
+                 disjoint(hi, n * sizeof(float), work, wb) && disjoint(lo, n * sizeof(float), work, wb),
+             "cf_rk4_tb_tune_pace: null or overlapping buffer");
+  hipStream_t st = S(stream);
+  TbGeom g = tb_f32c_geom(p->N, chunk);
+  float *ho = same_residue(work, hi), *lw = same_residue(ho + n, lo);
+  const TbStep k = tb_step_host(p, t, dt);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  CF_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    CF_REQUIRE(false, "cf_rk4_tb_tune_pace: hipEventCreate failed");
+  }
+  std::map<int, float> ms;  // pace -> ms per step
+  cf_status rc = CF_OK;
+  auto time_pace = [&](int pace) -> cf_status {
+    if (ms.count(pace)) return CF_OK;
+    g.pace = pace;
+    CF_TRY(tb_launch_sa(g, k, hi, lo, ho, lw, st));  // warm-up
+    CF_HIP(hipEventRecord(e0, st));
+    for (int r = 0; r < 3; ++r) CF_TRY(tb_launch_sa(g, k, hi, lo, ho, lw, st));
+    CF_HIP(hipEventRecord(e1, st));
+    CF_HIP(hipEventSynchronize(e1));
+    float el = 0.0f;
+    CF_HIP(hipEventElapsedTime(&el, e0, e1));
+    ms[pace] = el / 3.0f;
+    return CF_OK;
+  };
+  // coarse grid 0.64 .. 3.2 us per plane, then every tick around the coarse best
+  for (int pace = 0; pace <= 320 && rc == CF_OK; pace += (pace == 0 ? 64 : 8)) rc = time_pace(pace);
+  int coarse = 0;
+  for (const auto &kv : ms)
+    if (kv.second < ms[coarse]) coarse = kv.first;
+  if (coarse > 0)
+    for (int pace = coarse - 8; pace <= coarse + 8 && rc == CF_OK; ++pace) rc = time_pace(pace);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  CF_TRY(rc);
+  float best = ms[0];
+  for (const auto &kv : ms) best = kv.second < best ? kv.second : best;
+  int pick = 0;  // unpaced if that is within 1% of the fastest
+  if (ms[0] > best * 1.01f)
+    for (const auto &kv : ms)
+      if (kv.second <= best * 1.01f) pick = kv.first;  // ascending: the largest such pace
+  CF_TRY(tb_pace_store(p->N, g.A, pick));
+  if (pace_out) *pace_out = pick;
+  return CF_OK;
+}
+
+cf_status cf_rk4_tb_set_pace(int64_t N, int chunk, int pace) {
+  CF_REQUIRE(valid_N(N), "cf_rk4_tb_set_pace: unsupported lattice size N=%lld", (long long)N);
+  CF_REQUIRE(pace >= 0 && pace <= 100000, "cf_rk4_tb_set_pace: pace %d outside 0..100000", pace);
+  return tb_pace_store(N, tb_f32c_geom(N, chunk).A, pace);
+}
+
+int cf_rk4_tb_get_pace(int64_t N, int chunk) {
+  if (!valid_N(N)) return -1;
+  TbGeom g = tb_f32c_geom(N, chunk);
+  if (tb_pace_lookup(g) != CF_OK) return -1;
+  return g.pace;
 }
 
 }  // extern "C"

@@ -31,7 +31,11 @@
 // acc = 1*x + dt/6 k1 + dt/3 k2 + dt/3 k3 and ends with acc + dt/6 k4; the
 // compensated FP32 mode rounds y and the increment D to float after every
 // stage and folds (hi + lo) + (D + dt/6 k4) in double. The results are
-// therefore bit-identical to the four-kernel path.
+// therefore bit-identical to the four-kernel path. With SA (the single-precision
+// stencil, compensated storage only) each stage's force is float arithmetic on the
+// float stage inputs, y = hi + c k is formed in float and the increment D is kept
+// unrounded in double; the new state is folded as above. That scheme has no
+// four-kernel form; the oracle restates it (orc_rk4_step_f32s).
 #pragma once
 
 #include <type_traits>
@@ -51,6 +55,14 @@ struct TbStep {
   double m, fa2;  // equation variants: mass, f_a^2
 };
 
+// The scalars of the single-precision stencil (the compensated mode with float
+// right-hand sides): the RHS coefficients and the stage-input step lengths of
+// TbStep, rounded to float. The increment weights dt/6, dt/3 stay double.
+struct TbStepF {
+  float c_h[4], inv[4];
+  float c_m, lam, dh, dt;
+};
+
 // Equation of the step body: the Klein-Gordon family (+ lambda phi^3), or the
 // fixed-curvature / sqrt-potential variants in the operation order of their CUDA
 // kernels (Fixed_Curvature_FRW_equation_kernel, equations_cuda.cu:114-137;
@@ -66,6 +78,14 @@ struct TbPsi {
   int64_t G, sd, hs;
 };
 
+// Pacing. Neighbouring tiles read each other's edge rows and columns of every plane;
+// those reads hit the L2 only while the two blocks march through the same planes at
+// about the same time. Blocks that start together drift apart once an iteration's
+// compute is short against its memory latency, and their halo reads then go to HBM.
+// With PACE every wavefront of a block waits, before the barrier of iteration j, until
+// pace * j ticks of the 100 MHz real-time counter have passed since the block began,
+// so blocks cannot run ahead of the schedule (blocks that fall behind it catch up at
+// full speed). Pacing changes timing only, never results.
 // One work unit: lattice size, output tile origin and edges (T along b, Tc along
 // c), plane chunk, and the planes NA that the state buffers hold along a (N for a
 // periodic lattice; a slab of a larger lattice holds its planes plus ghost planes,
@@ -75,6 +95,7 @@ struct TbUnit {
   int64_t b0, c0, a0, a1;
   int T, Tc;
   int64_t NA;
+  int pace;  // ticks per iteration (bodies built with PACE)
 };
 
 __device__ __forceinline__ double tb_rhs(double phi, double pa1, double pam, double pb1, double pbm, double pc1,
@@ -84,6 +105,17 @@ __device__ __forceinline__ double tb_rhs(double phi, double pa1, double pam, dou
   if (has_lam) v = v - ((lam * phi) * phi) * phi;
   double r = v + inv * ((((-6.0 * phi + pa1) + pam) + pb1) + pbm);
   return r + inv * (pc1 + pcm);
+}
+
+// The Klein-Gordon force in float (single-precision stencil): the seven-point sum
+// in one chain (a, b, c neighbours), then one product with inv_ah_sqr.
+template <bool LAM>
+__device__ __forceinline__ float tb_rhs_f32(float phi, float pa1, float pam, float pb1, float pbm, float pc1,
+                                            float pcm, float pi, float c_h, float c_m, float lam, float inv) {
+  float v = c_h * pi - c_m * phi;
+  if constexpr (LAM) v = v - ((lam * phi) * phi) * phi;
+  const float s7 = (((((-6.0f * phi + pa1) + pam) + pb1) + pbm) + pc1) + pcm;
+  return v + inv * s7;
 }
 
 template <int VAR, bool LAM>
@@ -196,12 +228,14 @@ template <class T>
 using TbLds = T[2][4][TBR][TBR];
 
 template <class T, bool F32C, bool LAM, bool DEFER = true, bool LANE_NBR = true, int VAR = TB_KG, bool RING = false,
-          int RB = TBR, int RC = TBR>
+          int RB = TBR, int RC = TBR, bool SA = false, bool PACE = false>
 __device__ __forceinline__ void tb_unit(TbLds<T> &lds_all, const TbUnit &u, const TbStep &k,
                                         const T *__restrict__ xin, const float *__restrict__ lin,
                                         T *__restrict__ xout, float *__restrict__ lout,
-                                        const double *__restrict__ psi = nullptr, const TbPsi cps = TbPsi{}) {
+                                        const double *__restrict__ psi = nullptr, const TbPsi cps = TbPsi{},
+                                        const TbStepF kf32 = TbStepF{}) {
   static_assert(VAR == TB_KG || !F32C, "the equation variants are FP64");
+  static_assert(!SA || (F32C && VAR == TB_KG), "the single-precision stencil is a compensated Klein-Gordon mode");
   static_assert(RB * RC <= TBR * TBR && RB > 2 * TBH && RC > 2 * TBH, "region shape");
   static_assert(!LANE_NBR || RC == TBR, "lane neighbours need rows of 32 lanes");
   // the plane buffers of this shape, in the storage of the 32 x 32 ones
@@ -218,6 +252,10 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds_all, const TbUnit &u, cons
   // increment), so the histories are held as float (half the registers) and
   // widened where they are used; the arithmetic stays double.
   typedef std::conditional_t<F32C, float, double> Hv;
+  // Stage arithmetic (double; float with SA) and the increment's type (SA keeps the
+  // increment D unrounded in double).
+  typedef std::conditional_t<SA, float, double> Av;
+  typedef std::conditional_t<SA, double, Hv> Dv;
   const int64_t N = u.N, NN = N * N, NA = u.NA, N3 = NN * NA;  // N3: offset of the pi planes
   const int plane_bytes = (int)(NN * (int64_t)sizeof(T)), plane_bytes_f = (int)(NN * (int64_t)sizeof(float));
   int64_t b = (u.b0 - TBH + r) % N;
@@ -232,13 +270,13 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds_all, const TbUnit &u, cons
   // c+1 / c-1 neighbour of a stage input: from the adjacent lanes (LANE_NBR) or
   // from the LDS plane buffer (the same values; which is faster depends on the
   // kernel's mix of LDS and VALU work)
-  auto cnext = [&](Hv v, const T(*pl)[RC]) -> double {
-    if constexpr (LANE_NBR) return (double)tb_lane_next(v);
-    else return (double)pl[r][cp];
+  auto cnext = [&](Hv v, const T(*pl)[RC]) -> Av {
+    if constexpr (LANE_NBR) return (Av)tb_lane_next(v);
+    else return (Av)pl[r][cp];
   };
-  auto cprev = [&](Hv v, const T(*pl)[RC]) -> double {
-    if constexpr (LANE_NBR) return (double)tb_lane_prev(v);
-    else return (double)pl[r][cm];
+  auto cprev = [&](Hv v, const T(*pl)[RC]) -> Av {
+    if constexpr (LANE_NBR) return (Av)tb_lane_prev(v);
+    else return (Av)pl[r][cm];
   };
 
   // plane offsets (times N^2) advanced by one plane per iteration, wrapping at N
@@ -268,7 +306,7 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds_all, const TbUnit &u, cons
   constexpr int TBU = 8;
   Hv XF[TBU] = {}, XP[TBU] = {};
   Hv Y1F[TBU] = {}, Y1P[TBU] = {}, Y2F[TBU] = {}, Y2P[TBU] = {}, Y3F[TBU] = {}, Y3P[TBU] = {};
-  Hv AF[TBU] = {}, AP[TBU] = {};
+  Dv AF[TBU] = {}, AP[TBU] = {};
   double PS[TBU] = {};  // fixed curvature: Psi rides with x (same ring, same planes)
 
   const int iters = (int)(u.a1 - u.a0) + 8;
@@ -340,6 +378,7 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds_all, const TbUnit &u, cons
     }
   };
 
+  const uint64_t t_begin = PACE ? __builtin_amdgcn_s_memrealtime() : 0;
   auto iteration = [&](auto phase, auto slot, const int j) {
     constexpr int PH = decltype(phase)::value;
     constexpr int I0 = PH % TBU, I1 = (PH + TBU - 1) % TBU, I2 = (PH + TBU - 2) % TBU;
@@ -374,6 +413,10 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds_all, const TbUnit &u, cons
       buf[2][r][cc] = (T)Y2F[I1];
       buf[3][r][cc] = (T)Y3F[I1];
     }
+    if constexpr (PACE) {
+      const uint64_t due = t_begin + (uint64_t)j * (uint64_t)u.pace;
+      while (__builtin_amdgcn_s_memrealtime() < due) __builtin_amdgcn_s_sleep(1);
+    }
     __syncthreads();
 
     // the stages' Psi: fixed curvature from its plane history, comoving from the
@@ -391,57 +434,97 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds_all, const TbUnit &u, cons
       cpsi(4, 2, ps4, dp4);
     }
 
-    // ---- stage 1 at P-1: k1 = f(x)
-    double kf = (double)XP[I1];
-    double kp = tb_force<VAR, LAM>((double)XF[I1], (double)XF[I0], (double)XF[I2], (double)buf[0][rp][cc],
-                                  (double)buf[0][rm][cc], cnext(XF[I1], buf[0]), cprev(XF[I1], buf[0]),
-                                  (double)XP[I1], k.c_h[0], k.inv[0], k, ps1, dp1);
-    Y1F[I0] = (Hv)rnd<F32C>((double)XF[I1] + k.dh * kf);
-    Y1P[I0] = (Hv)rnd<F32C>((double)XP[I1] + k.dh * kp);
-    if constexpr (F32C) {
-      AF[I0] = (Hv)(k.dt6 * kf);
-      AP[I0] = (Hv)(k.dt6 * kp);
-    } else {
-      AF[I0] = XF[I1] + k.dt6 * kf;
-      AP[I0] = XP[I1] + k.dt6 * kp;
-    }
+    if constexpr (SA) {
+      // Single-precision stencil: each stage's force in float on the float stage
+      // inputs, y = hi + c k in float, the increment D = dt/6 k1 + dt/3 k2 + dt/3 k3
+      // in double, and the new state (hi + lo) + (D + dt/6 k4) in double.
+      float kf = XP[I1];
+      float kp = tb_rhs_f32<LAM>(XF[I1], XF[I0], XF[I2], buf[0][rp][cc], buf[0][rm][cc], cnext(XF[I1], buf[0]),
+                                 cprev(XF[I1], buf[0]), XP[I1], kf32.c_h[0], kf32.c_m, kf32.lam, kf32.inv[0]);
+      Y1F[I0] = XF[I1] + kf32.dh * kf;
+      Y1P[I0] = XP[I1] + kf32.dh * kp;
+      AF[I0] = k.dt6 * (double)kf;
+      AP[I0] = k.dt6 * (double)kp;
 
-    // ---- stage 2 at P-2: k2 = f(y1)
-    kf = (double)Y1P[I1];
-    kp = tb_force<VAR, LAM>((double)Y1F[I1], (double)Y1F[I0], (double)Y1F[I2], (double)buf[1][rp][cc],
-                            (double)buf[1][rm][cc], cnext(Y1F[I1], buf[1]), cprev(Y1F[I1], buf[1]),
-                            (double)Y1P[I1], k.c_h[1], k.inv[1], k, ps2, dp2);
-    Y2F[I0] = (Hv)rnd<F32C>((double)XF[I2] + k.dh * kf);
-    Y2P[I0] = (Hv)rnd<F32C>((double)XP[I2] + k.dh * kp);
-    AF[I1] = (Hv)((double)AF[I1] + k.dt3 * kf);
-    AP[I1] = (Hv)((double)AP[I1] + k.dt3 * kp);
+      kf = Y1P[I1];
+      kp = tb_rhs_f32<LAM>(Y1F[I1], Y1F[I0], Y1F[I2], buf[1][rp][cc], buf[1][rm][cc], cnext(Y1F[I1], buf[1]),
+                           cprev(Y1F[I1], buf[1]), Y1P[I1], kf32.c_h[1], kf32.c_m, kf32.lam, kf32.inv[1]);
+      Y2F[I0] = XF[I2] + kf32.dh * kf;
+      Y2P[I0] = XP[I2] + kf32.dh * kp;
+      AF[I1] = AF[I1] + k.dt3 * (double)kf;
+      AP[I1] = AP[I1] + k.dt3 * (double)kp;
 
-    // ---- stage 3 at P-3: k3 = f(y2)
-    kf = (double)Y2P[I1];
-    kp = tb_force<VAR, LAM>((double)Y2F[I1], (double)Y2F[I0], (double)Y2F[I2], (double)buf[2][rp][cc],
-                            (double)buf[2][rm][cc], cnext(Y2F[I1], buf[2]), cprev(Y2F[I1], buf[2]),
-                            (double)Y2P[I1], k.c_h[2], k.inv[2], k, ps3, dp3);
-    Y3F[I0] = (Hv)rnd<F32C>((double)XF[I3] + k.dt * kf);
-    Y3P[I0] = (Hv)rnd<F32C>((double)XP[I3] + k.dt * kp);
-    AF[I2] = (Hv)((double)AF[I2] + k.dt3 * kf);
-    AP[I2] = (Hv)((double)AP[I2] + k.dt3 * kp);
+      kf = Y2P[I1];
+      kp = tb_rhs_f32<LAM>(Y2F[I1], Y2F[I0], Y2F[I2], buf[2][rp][cc], buf[2][rm][cc], cnext(Y2F[I1], buf[2]),
+                           cprev(Y2F[I1], buf[2]), Y2P[I1], kf32.c_h[2], kf32.c_m, kf32.lam, kf32.inv[2]);
+      Y3F[I0] = XF[I3] + kf32.dt * kf;
+      Y3P[I0] = XP[I3] + kf32.dt * kp;
+      AF[I2] = AF[I2] + k.dt3 * (double)kf;
+      AP[I2] = AP[I2] + k.dt3 * (double)kp;
 
-    // ---- stage 4 at P-4: k4 = f(y3); new state
-    kf = (double)Y3P[I1];
-    kp = tb_force<VAR, LAM>((double)Y3F[I1], (double)Y3F[I0], (double)Y3F[I2], (double)buf[3][rp][cc],
-                            (double)buf[3][rm][cc], cnext(Y3F[I1], buf[3]), cprev(Y3F[I1], buf[3]),
-                            (double)Y3P[I1], k.c_h[3], k.inv[3], k, ps4, dp4);
-    if constexpr (F32C) {
-      const double s0 = ((double)XF[I4] + lo_f) + ((double)AF[I3] + k.dt6 * kf);
-      const double s1 = ((double)XP[I4] + lo_p) + ((double)AP[I3] + k.dt6 * kp);
+      kf = Y3P[I1];
+      kp = tb_rhs_f32<LAM>(Y3F[I1], Y3F[I0], Y3F[I2], buf[3][rp][cc], buf[3][rm][cc], cnext(Y3F[I1], buf[3]),
+                           cprev(Y3F[I1], buf[3]), Y3P[I1], kf32.c_h[3], kf32.c_m, kf32.lam, kf32.inv[3]);
+      const double s0 = ((double)XF[I4] + lo_f) + (AF[I3] + k.dt6 * (double)kf);
+      const double s1 = ((double)XP[I4] + lo_p) + (AP[I3] + k.dt6 * (double)kp);
       const float h0 = (float)s0, h1 = (float)s1;
       q.f = h0;
       q.p = h1;
       q.lf = (float)(s0 - (double)h0);
       q.lp = (float)(s1 - (double)h1);
     } else {
-      q.f = AF[I3] + k.dt6 * kf;
-      q.p = AP[I3] + k.dt6 * kp;
+      // ---- stage 1 at P-1: k1 = f(x)
+      double kf = (double)XP[I1];
+      double kp = tb_force<VAR, LAM>((double)XF[I1], (double)XF[I0], (double)XF[I2], (double)buf[0][rp][cc],
+                                    (double)buf[0][rm][cc], cnext(XF[I1], buf[0]), cprev(XF[I1], buf[0]),
+                                    (double)XP[I1], k.c_h[0], k.inv[0], k, ps1, dp1);
+      Y1F[I0] = (Hv)rnd<F32C>((double)XF[I1] + k.dh * kf);
+      Y1P[I0] = (Hv)rnd<F32C>((double)XP[I1] + k.dh * kp);
+      if constexpr (F32C) {
+        AF[I0] = (Hv)(k.dt6 * kf);
+        AP[I0] = (Hv)(k.dt6 * kp);
+      } else {
+        AF[I0] = XF[I1] + k.dt6 * kf;
+        AP[I0] = XP[I1] + k.dt6 * kp;
+      }
+
+      // ---- stage 2 at P-2: k2 = f(y1)
+      kf = (double)Y1P[I1];
+      kp = tb_force<VAR, LAM>((double)Y1F[I1], (double)Y1F[I0], (double)Y1F[I2], (double)buf[1][rp][cc],
+                              (double)buf[1][rm][cc], cnext(Y1F[I1], buf[1]), cprev(Y1F[I1], buf[1]),
+                              (double)Y1P[I1], k.c_h[1], k.inv[1], k, ps2, dp2);
+      Y2F[I0] = (Hv)rnd<F32C>((double)XF[I2] + k.dh * kf);
+      Y2P[I0] = (Hv)rnd<F32C>((double)XP[I2] + k.dh * kp);
+      AF[I1] = (Hv)((double)AF[I1] + k.dt3 * kf);
+      AP[I1] = (Hv)((double)AP[I1] + k.dt3 * kp);
+
+      // ---- stage 3 at P-3: k3 = f(y2)
+      kf = (double)Y2P[I1];
+      kp = tb_force<VAR, LAM>((double)Y2F[I1], (double)Y2F[I0], (double)Y2F[I2], (double)buf[2][rp][cc],
+                              (double)buf[2][rm][cc], cnext(Y2F[I1], buf[2]), cprev(Y2F[I1], buf[2]),
+                              (double)Y2P[I1], k.c_h[2], k.inv[2], k, ps3, dp3);
+      Y3F[I0] = (Hv)rnd<F32C>((double)XF[I3] + k.dt * kf);
+      Y3P[I0] = (Hv)rnd<F32C>((double)XP[I3] + k.dt * kp);
+      AF[I2] = (Hv)((double)AF[I2] + k.dt3 * kf);
+      AP[I2] = (Hv)((double)AP[I2] + k.dt3 * kp);
+
+      // ---- stage 4 at P-4: k4 = f(y3); new state
+      kf = (double)Y3P[I1];
+      kp = tb_force<VAR, LAM>((double)Y3F[I1], (double)Y3F[I0], (double)Y3F[I2], (double)buf[3][rp][cc],
+                              (double)buf[3][rm][cc], cnext(Y3F[I1], buf[3]), cprev(Y3F[I1], buf[3]),
+                              (double)Y3P[I1], k.c_h[3], k.inv[3], k, ps4, dp4);
+      if constexpr (F32C) {
+        const double s0 = ((double)XF[I4] + lo_f) + ((double)AF[I3] + k.dt6 * kf);
+        const double s1 = ((double)XP[I4] + lo_p) + ((double)AP[I3] + k.dt6 * kp);
+        const float h0 = (float)s0, h1 = (float)s1;
+        q.f = h0;
+        q.p = h1;
+        q.lf = (float)(s0 - (double)h0);
+        q.lp = (float)(s1 - (double)h1);
+      } else {
+        q.f = AF[I3] + k.dt6 * kf;
+        q.p = AP[I3] + k.dt6 * kp;
+      }
     }
     q.off = st_off;
     q.plane = off_st;

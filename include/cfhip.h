@@ -121,6 +121,7 @@ cf_status cf_count_nonfinite(const double *v_dev, int64_t n, int64_t *count_host
  * work_dev holds cf_rk4_work_bytes(N, CF_PREC_F64) bytes. */
 #define CF_PREC_F64 0
 #define CF_PREC_F32C 1 /* float hi+lo state, float stage arrays, double arithmetic */
+#define CF_PREC_F32S 2 /* float hi+lo state, float stencil arithmetic, double increment */
 size_t cf_rk4_work_bytes(int64_t N, int precision);
 cf_status cf_rk4_step_f64(const cf_kg_params *p, double *x_dev, double t, double dt, void *work_dev,
                           cf_stream stream);
@@ -147,6 +148,31 @@ cf_status cf_rk4_run_tb_f64(const cf_kg_params *p, double *x_dev, double t0, int
                             double dt, int chunk, void *work_dev, cf_stream stream);
 cf_status cf_rk4_run_tb_f32c(const cf_kg_params *p, float *hi_dev, float *lo_dev, double t0, int64_t k0,
                              int64_t n_steps, double dt, int chunk, void *work_dev, cf_stream stream);
+/* Compensated storage with the single-precision stencil (no reference counterpart;
+ * the run-A storage of the precision study with float right-hand sides): the state
+ * is hi + lo as in cf_rk4_run_tb_f32c, each stage's force is evaluated in float on
+ * the float stage inputs (seven-point sum, then the coefficients rounded to float),
+ * the stage inputs are hi + c k in float, the increment dt/6 k1 + dt/3 k2 + dt/3 k3 is
+ * accumulated in double and the new state (hi + lo) + (increment + dt/6 k4) is formed
+ * in double and split into hi and lo. The increment weights stay double, so the
+ * scheme has no drift from rounded step lengths. Work buffer and placement as
+ * cf_rk4_run_tb_f32c (cf_rk4_tb_work_bytes(N, CF_PREC_F32S)). */
+cf_status cf_rk4_run_tb_f32s(const cf_kg_params *p, float *hi_dev, float *lo_dev, double t0, int64_t k0,
+                             int64_t n_steps, double dt, int chunk, void *work_dev, cf_stream stream);
+/* Pacing of the single-precision-stencil step (no reference counterpart). Its blocks
+ * run faster when held to a common schedule of plane iterations (neighbouring tiles
+ * then read their shared halo rows while these are still in the L2). The pace, in ticks
+ * of the GPU's 100 MHz real-time counter per plane iteration (0: unpaced), is kept per
+ * device, lattice size and chunk and used by cf_rk4_run_tb_f32s and cf_rk4_step_tb_f32s.
+ * cf_rk4_tb_tune_pace times about 50 steps of the lattice of p at paces 0..3.2 us per
+ * plane (state hi/lo at time t, unchanged; work as for cf_rk4_run_tb_f32s), keeps the
+ * largest pace within 1% of the fastest (unpaced if that is within 1%), and returns it
+ * in *pace_out (may be null); it synchronises the stream. Pacing changes timing only,
+ * never results. cf_rk4_tb_get_pace returns -1 for an unsupported N. */
+cf_status cf_rk4_tb_tune_pace(const cf_kg_params *p, const float *hi_dev, const float *lo_dev, double t, double dt,
+                              int chunk, void *work_dev, cf_stream stream, int *pace_out);
+cf_status cf_rk4_tb_set_pace(int64_t N, int chunk, int pace);
+int cf_rk4_tb_get_pace(int64_t N, int chunk);
 /* cf_rk4_run_tb_f64 without a work buffer and without any allocation. No counterpart
  * in the reference: it is for callers that cannot allocate device memory (code
  * under hipGraph capture, serving loops, harnesses that forbid hipMalloc). The
@@ -164,6 +190,9 @@ cf_status cf_rk4_run_tb_f64_noalloc(const cf_kg_params *p, double *x_dev, double
 cf_status cf_rk4_step_tb_f64(const cf_kg_params *p, const double *x_in_dev, double *x_out_dev, double t, double dt,
                              int chunk, cf_stream stream);
 cf_status cf_rk4_step_tb_f32c(const cf_kg_params *p, const float *hi_in_dev, const float *lo_in_dev,
+                              float *hi_out_dev, float *lo_out_dev, double t, double dt, int chunk,
+                              cf_stream stream);
+cf_status cf_rk4_step_tb_f32s(const cf_kg_params *p, const float *hi_in_dev, const float *lo_in_dev,
                               float *hi_out_dev, float *lo_out_dev, double t, double dt, int chunk,
                               cf_stream stream);
 /* One temporally blocked FP64 step of a slab of an N^3 lattice split along a: the

@@ -255,6 +255,75 @@ extern "C" void orc_rk4_step_f32c(const OrcEq *e, float *hi, float *lo, double t
   }
 }
 
+// The compensated storage with the single-precision stencil (no reference
+// counterpart; the arithmetic csrc/cf_tb_body.hpp defines for it, restated so the GPU
+// path can be checked bit for bit). State = hi + lo, both float. Each stage's force is
+// float arithmetic on float inputs with the coefficients rounded to float:
+//   v = (-3H)f * pi - (m*m)f * phi;  v -= (((lambda f) * phi) * phi) * phi  (lambda != 0)
+//   s7 = (((((-6 phi + phi_a+) + phi_a-) + phi_b+) + phi_b-) + phi_c+) + phi_c-
+//   dpi = v + (inv_ah_sqr)f * s7;  dphi = pi
+// the stage inputs y = hi + c k in float (c = (dt/2)f, (dt/2)f, (dt)f), the increment
+// D = dt/6 k1 + dt/3 k2 + dt/3 k3 in double, and
+//   s = (hi + lo) + (D + dt/6 k4);  hi = f32(s);  lo = f32(s - hi)   (double).
+static void rhs_f32(const OrcEq *e, const float *y, float *k, double t) {
+  const i64 N = e->N, N3 = N * N * N;
+  const double a_t = orc_cosmo_a(&e->cosmo, t);
+  const float c_h = (float)((-3.0) * orc_cosmo_H(&e->cosmo, t));
+  const float inv = (float)orc_inv_ah_sqr(N, e->L, a_t);
+  const float c_m = (float)(e->m * e->m), lam = (float)e->lambda;
+  const bool has_lam = e->lambda != 0.0;
+  parallel_for(N, [&](i64 a) {
+    for (i64 b = 0; b < N; ++b) {
+      const float *p0 = y + IDX(N, a, b, 0);
+      const float *pa1 = y + IDX(N, (a + 1) % N, b, 0);
+      const float *pam = y + IDX(N, (a + N - 1) % N, b, 0);
+      const float *pb1 = y + IDX(N, a, (b + 1) % N, 0);
+      const float *pbm = y + IDX(N, a, (b + N - 1) % N, 0);
+      const float *pi = y + N3 + IDX(N, a, b, 0);
+      float *dphi = k + IDX(N, a, b, 0);
+      float *dpi = k + N3 + IDX(N, a, b, 0);
+      for (i64 c = 0; c < N; ++c) {
+        const float phi = p0[c];
+        const float pc1 = p0[(c + 1) % N], pcm = p0[(c + N - 1) % N];
+        float v = c_h * pi[c] - c_m * phi;
+        if (has_lam) v = v - ((lam * phi) * phi) * phi;
+        const float s7 = (((((-6.0f * phi + pa1[c]) + pam[c]) + pb1[c]) + pbm[c]) + pc1) + pcm;
+        dphi[c] = pi[c];
+        dpi[c] = v + inv * s7;
+      }
+    }
+  });
+}
+
+extern "C" void orc_rk4_step_f32s(const OrcEq *e, float *hi, float *lo, double t, double dt,
+                                  double *work /* 2N^3 doubles */, float *fwork /* 2 * 2N^3 floats */) {
+  const i64 n = 2 * e->N * e->N * e->N;
+  double *D = work;
+  float *y = fwork, *k = fwork + n;
+  const double dh = dt / 2.0, th = t + dh, dt6 = dt / 6.0, dt3 = dt / 3.0;
+  const float cy[3] = {(float)dh, (float)dh, (float)dt};
+  const double ts[4] = {t, th, th, t + dt};
+  rhs_f32(e, hi, k, ts[0]);
+  for (i64 i = 0; i < n; ++i) {
+    D[i] = dt6 * (double)k[i];
+    y[i] = hi[i] + cy[0] * k[i];
+  }
+  for (int s = 1; s <= 2; ++s) {
+    rhs_f32(e, y, k, ts[s]);
+    for (i64 i = 0; i < n; ++i) {
+      D[i] = D[i] + dt3 * (double)k[i];
+      y[i] = hi[i] + cy[s] * k[i];
+    }
+  }
+  rhs_f32(e, y, k, ts[3]);
+  for (i64 i = 0; i < n; ++i) {
+    const double s = ((double)hi[i] + (double)lo[i]) + (D[i] + dt6 * (double)k[i]);
+    const float h = (float)s;
+    hi[i] = h;
+    lo[i] = (float)(s - (double)h);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // FFT: a mixed-radix Stockham FFT written for the oracle (FFTW is absent from
 // this image). Conventions follow FFTW's r2c/c2r 3-D transforms as used by

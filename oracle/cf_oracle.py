@@ -74,6 +74,7 @@ def lib():
         L.orc_rk4_run.argtypes = [C.POINTER(Eq), _dp, C.c_double, C.c_double, C.c_int, _dp]
         L.orc_rk4_run.restype = C.c_int
         L.orc_rk4_step_f32c.argtypes = [C.POINTER(Eq), _fp, _fp, C.c_double, C.c_double, _dp, _fp]
+        L.orc_rk4_step_f32s.argtypes = [C.POINTER(Eq), _fp, _fp, C.c_double, C.c_double, _dp, _fp]
         L.orc_fft_r2c.argtypes = [C.c_int64, _dp, _dp]
         L.orc_fft_c2r.argtypes = [C.c_int64, _dp, _dp]
         L.orc_dft_r2c_direct.argtypes = [C.c_int64, _dp, _dp]
@@ -195,6 +196,13 @@ class Equation:
         fw = np.empty(2 * n, np.float32)
         lib().orc_rk4_step_f32c(C.byref(self._eq), _p(hi, np.float32), _p(lo, np.float32), t, dt,
                                 _p(self._w(2)), _p(fw, np.float32))
+
+    def rk4_step_f32s(self, hi, lo, t, dt):
+        """The single-precision-stencil compensated step (cf_oracle.cpp orc_rk4_step_f32s)."""
+        n = 2 * self.N ** 3
+        fw = np.empty(2 * n, np.float32)
+        lib().orc_rk4_step_f32s(C.byref(self._eq), _p(hi, np.float32), _p(lo, np.float32), t, dt,
+                                _p(self._w(1)), _p(fw, np.float32))
 
 
 def less_eq_with_sign(t1, t2, dt):

@@ -77,6 +77,7 @@ def test_sizes(lib, O):
     # temporally blocked steps ping-pong with one more state (FP32-comp: hi and lo)
     assert L.cf_rk4_tb_work_bytes(512, 0) == 2 * 512 ** 3 * 8
     assert L.cf_rk4_tb_work_bytes(512, 1) == 2 * 2 * 512 ** 3 * 4 + 2 * 4096  # + slack to align the second pair
+    assert L.cf_rk4_tb_work_bytes(512, lib.CF_PREC_F32S) == L.cf_rk4_tb_work_bytes(512, 1)  # same storage
     # adjoint: checkpoints + one segment + step work (2) + lambda (2) + forward work
     # (3) state vectors, and 4 dL/dm partials per block (512 blocks) per step plus one
     v = 2 * 128 ** 3 * 8
@@ -119,6 +120,17 @@ def test_bad_arguments_rejected_before_device_work(lib):
         lib.CF_E_INVALID  # overlapping in/out
     assert L.cf_rk4_step_tb_f32c(C.byref(p), x0, C.c_void_p(0x900000), x0, C.c_void_p(0xA00000), 1.0, 0.1, 0,
                                  None) == lib.CF_E_INVALID
+    assert L.cf_rk4_step_tb_f32s(C.byref(p), x0, C.c_void_p(0x900000), x0, C.c_void_p(0xA00000), 1.0, 0.1, 0,
+                                 None) == lib.CF_E_INVALID
+    assert b"cf_rk4_step_tb_f32s" in L.cf_last_error()
+    assert L.cf_rk4_run_tb_f32s(C.byref(p), x0, None, 1.0, 0, 1, 0.1, 0, fake, None) == lib.CF_E_INVALID
+    # pacing of the single-precision-stencil step: bad lattice, bad pace, null buffers
+    assert L.cf_rk4_tb_set_pace(15, 0, 100) == lib.CF_E_INVALID
+    assert L.cf_rk4_tb_set_pace(16, 0, -1) == lib.CF_E_INVALID
+    assert L.cf_rk4_tb_get_pace(15, 0) == -1
+    pace = C.c_int(7)
+    assert L.cf_rk4_tb_tune_pace(C.byref(p), None, x0, 1.0, 0.1, 0, fake, None, C.byref(pace)) == lib.CF_E_INVALID
+    assert pace.value == 7
     pw = lib.KGParams(16, 16.0, 1.0, 0.0, lib.Cosmology(1.0, 0.5, 3.0, 1.0))  # t1 != 1/(2 H1)
     assert L.cf_wkb_create(C.byref(pw), fake, 3.0, C.byref(h), None) == lib.CF_E_INVALID
     assert b"radiation" in L.cf_last_error()

@@ -5,7 +5,8 @@
   stores phi and dphi/dt as FP32 hi + FP32 compensation term (FP64 arithmetic), run B
   is FP64; both use the temporally blocked step. Every 100 steps the field (mode)
   spectrum and the energy-density spectrum of A must lie within 1e-6 relative of B's,
-  and the states within 1e-6 (relative L2).
+  and the states within 1e-6 (relative L2). Run A in both compensated forms: FP64
+  arithmetic (f32c) and float stencil arithmetic with a double increment (f32s).
 * The ensemble: 96 realisations, 32 each at N=64, 96, 128 (L=N, seeds 100-195 in
   that order), FRW Klein-Gordon with m=1 and lambda=0.1, FP32-comp, P(k) ~
   exp(-k^2/(2*0.5^2)), rms 0.5, dt=0.05, the autotuned plan and graph replay, for 5
@@ -38,9 +39,11 @@ def rel_bins(a, b):
     return float(d.max() / peak), float((d[sig] / b[sig].abs()).max())
 
 
-def test_precision_run_a_vs_run_b_n512(cfl):
+@pytest.mark.parametrize("mode", ["f32c", "f32s"])
+def test_precision_run_a_vs_run_b_n512(cfl, mode):
     N, L, m, dt, t0 = 512, 512.0, 1.0, 0.05, 1.0
-    print(f"case precision N={N} 1000 steps", flush=True)
+    print(f"case precision {mode} N={N} 1000 steps", flush=True)
+    run_a = cfl.rk4_run_tb_f32c if mode == "f32c" else cfl.rk4_run_tb_f32s
     cosmo = cfl.radiation(**FRW)
     p = cfl.kg_params(N, L, m, 0.0, cosmo)
     plan = cfl.SpectrumPlan(N)
@@ -59,7 +62,7 @@ def test_precision_run_a_vs_run_b_n512(cfl):
 
     rows = []
     for done in range(0, 1000, 100):
-        cfl.rk4_run_tb_f32c(p, hi, lo, t0, done, 100, dt, wA)
+        run_a(p, hi, lo, t0, done, 100, dt, wA)
         cfl.rk4_run_tb_f64(p, xB, t0, done, 100, dt, wB)
         t = t0 + (done + 100) * dt
         xA = cfl.join_f64(hi, lo)

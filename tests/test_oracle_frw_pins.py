@@ -130,3 +130,95 @@ def test_mode_spectrum_two_modes_at_a(O, a_t):
 
 def test_mode_spectrum_planted_slip_fails(O_planted):
     assert two_mode_spectrum_error(O_planted, 8, 8.0, 0.5, 2.5) > 1e-3
+
+
+# The compensated storage with the single-precision stencil (orc_rk4_step_f32s): the
+# same closed forms, to the precision of float forces on float stage inputs.
+def frw_plane_wave_error_f32s(O, axis, N, L, m, k):
+    cosmo = O.Cosmo.radiation(1.0, 0.5, 1.0)
+    eq = O.Equation(N, L, m, 0.0, cosmo)
+    pattern = np.cos(2 * np.pi * k * np.indices((N, N, N))[axis] / N).ravel()
+    A, B = 0.8, -0.3
+    x = np.concatenate([A * pattern, B * pattern])
+    hi = x.astype(np.float32)
+    lo = (x - hi.astype(np.float64)).astype(np.float32)
+    t0, dt, n = 4.0, 0.05, 40
+    for s in range(n):
+        eq.rk4_step_f32s(hi, lo, t0 + s * dt, dt)
+    xs = hi.astype(np.float64) + lo.astype(np.float64)
+    h = L / N
+    w2 = 4.0 * math.sin(math.pi * k / N) ** 2 / (h * h)
+    y = scalar_mode_rk4(A, B, t0, dt, n, m, w2, lambda t: O.cosmo_H(cosmo, t), lambda t: O.cosmo_a(cosmo, t))
+    return max(np.max(np.abs(xs[: N ** 3] - y[0] * pattern)), np.max(np.abs(xs[N ** 3:] - y[1] * pattern)))
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("N,L,m,k", [(8, 8.0, 0.5, 1), (12, 9.0, 0.3, 2)])
+def test_f32s_frw_plane_wave_rk4(O, axis, N, L, m, k):
+    err = frw_plane_wave_error_f32s(O, axis, N, L, m, k)
+    assert err < 1e-7, err
+    assert err > 0.0  # float forces: not the FP64 result
+
+
+def test_f32s_frw_plane_wave_planted_slip_fails(O_planted):
+    for axis in (0, 1, 2):
+        assert frw_plane_wave_error_f32s(O_planted, axis, 8, 8.0, 0.5, 1) > 1e-4
+
+
+def test_f32s_homogeneous_lambda(O):
+    """A homogeneous field with lambda phi^4/4 in a radiation era obeys
+    phi'' = -3 H phi' - m^2 phi - lambda phi^3 (no gradient); the f32s step follows the
+    classic RK4 of that scalar ODE to float precision, and an equation with lambda
+    dropped does not."""
+    N, L, m, lam = 8, 8.0, 0.7, 0.1
+    cosmo = O.Cosmo.radiation(1.0, 0.5, 1.0)
+    H = lambda t: O.cosmo_H(cosmo, t)  # noqa: E731
+
+    def scalar(f, g, t0, dt, n, lam_):
+        def F(y, t):
+            return np.array([y[1], -3.0 * H(t) * y[1] - m * m * y[0] - lam_ * y[0] ** 3])
+        y = np.array([f, g])
+        for s in range(n):
+            t = t0 + s * dt
+            k1 = F(y, t)
+            k2 = F(y + dt / 2 * k1, t + dt / 2)
+            k3 = F(y + dt / 2 * k2, t + dt / 2)
+            k4 = F(y + dt * k3, t + dt)
+            y = y + dt / 6 * k1 + dt / 3 * k2 + dt / 3 * k3 + dt / 6 * k4
+        return y
+
+    f0, g0, t0, dt, n = 1.3, -0.2, 2.0, 0.05, 60
+    x = np.concatenate([np.full(N ** 3, f0), np.full(N ** 3, g0)])
+    hi = x.astype(np.float32)
+    lo = (x - hi.astype(np.float64)).astype(np.float32)
+    eq = O.Equation(N, L, m, lam, cosmo)
+    for s in range(n):
+        eq.rk4_step_f32s(hi, lo, t0 + s * dt, dt)
+    xs = hi.astype(np.float64) + lo.astype(np.float64)
+    y = scalar(f0, g0, t0, dt, n, lam)
+    y0 = scalar(f0, g0, t0, dt, n, 0.0)
+    err = max(np.max(np.abs(xs[: N ** 3] - y[0])), np.max(np.abs(xs[N ** 3:] - y[1])))
+    assert err < 1e-7, err
+    assert max(abs(y[0] - y0[0]), abs(y[1] - y0[1])) > 1e-2  # the lambda term is visible
+
+
+def test_f32s_tracks_fp64_random_field(O):
+    """Over 100 FRW steps of a random field (lambda = 0.1) the f32s state stays within
+    1e-6 of FP64 (relative L2) and within 2x the deviation of the FP64-arithmetic
+    compensated step."""
+    N, L = 16, 16.0
+    x = O.initial_state(N, L, 9, O.SPEC_GAUSS, (0.5,), 0.5)
+    eq = O.Equation(N, L, 1.0, 0.1, O.Cosmo.radiation(1.0, 0.5, 1.0))
+    ref = x.copy()
+    hi = x.astype(np.float32)
+    lo = (x - hi.astype(np.float64)).astype(np.float32)
+    hc, lc = hi.copy(), lo.copy()
+    t0, dt = 1.0, 0.05
+    for s in range(100):
+        t = t0 + s * dt
+        eq.rk4_step(ref, t, dt)
+        eq.rk4_step_f32s(hi, lo, t, dt)
+        eq.rk4_step_f32c(hc, lc, t, dt)
+    es = np.linalg.norm(hi.astype(np.float64) + lo - ref) / np.linalg.norm(ref)
+    ec = np.linalg.norm(hc.astype(np.float64) + lc - ref) / np.linalg.norm(ref)
+    assert es < 1e-6 and es < 2 * ec, (es, ec)
