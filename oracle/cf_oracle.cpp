@@ -258,13 +258,14 @@ extern "C" void orc_rk4_step_f32c(const OrcEq *e, float *hi, float *lo, double t
 // The compensated storage with the single-precision stencil (no reference
 // counterpart; the arithmetic csrc/cf_tb_body.hpp defines for it, restated so the GPU
 // path can be checked bit for bit). State = hi + lo, both float. Each stage's force is
-// float arithmetic on float inputs with the coefficients rounded to float:
-//   v = (-3H)f * pi - (m*m)f * phi;  v -= (((lambda f) * phi) * phi) * phi  (lambda != 0)
-//   s7 = (((((-6 phi + phi_a+) + phi_a-) + phi_b+) + phi_b-) + phi_c+) + phi_c-
-//   dpi = v + (inv_ah_sqr)f * s7;  dphi = pi
-// the stage inputs y = hi + c k in float (c = (dt/2)f, (dt/2)f, (dt)f), the increment
-// D = dt/6 k1 + dt/3 k2 + dt/3 k3 in double, and
-//   s = (hi + lo) + (D + dt/6 k4);  hi = f32(s);  lo = f32(s - hi)   (double).
+// float arithmetic on float inputs with the coefficients rounded to float (fma: one
+// rounding):
+//   v = fma((-3H)f, pi, -((m*m)f * phi));  v = fma(-(((lambda f) * phi) * phi), phi, v)
+//   s7 = ((((fma(-6, phi, phi_a+) + phi_a-) + phi_b+) + phi_b-) + phi_c+) + phi_c-
+//   dpi = fma((inv_ah_sqr)f, s7, v);  dphi = pi
+// the stage inputs y = fma(c, k, hi) in float (c = (dt/2)f, (dt/2)f, (dt)f), the
+// increment D = dt/6 k1, then D = fma(dt/3, k, D) twice, in double, and
+//   s = (hi + lo) + fma(dt/6, k4, D);  hi = f32(s);  lo = f32(s - hi)   (double).
 static void rhs_f32(const OrcEq *e, const float *y, float *k, double t) {
   const i64 N = e->N, N3 = N * N * N;
   const double a_t = orc_cosmo_a(&e->cosmo, t);
@@ -285,11 +286,11 @@ static void rhs_f32(const OrcEq *e, const float *y, float *k, double t) {
       for (i64 c = 0; c < N; ++c) {
         const float phi = p0[c];
         const float pc1 = p0[(c + 1) % N], pcm = p0[(c + N - 1) % N];
-        float v = c_h * pi[c] - c_m * phi;
-        if (has_lam) v = v - ((lam * phi) * phi) * phi;
-        const float s7 = (((((-6.0f * phi + pa1[c]) + pam[c]) + pb1[c]) + pbm[c]) + pc1) + pcm;
+        float v = std::fmaf(c_h, pi[c], -(c_m * phi));
+        if (has_lam) v = std::fmaf(-((lam * phi) * phi), phi, v);
+        const float s7 = ((((std::fmaf(-6.0f, phi, pa1[c]) + pam[c]) + pb1[c]) + pbm[c]) + pc1) + pcm;
         dphi[c] = pi[c];
-        dpi[c] = v + inv * s7;
+        dpi[c] = std::fmaf(inv, s7, v);
       }
     }
   });
@@ -306,18 +307,18 @@ extern "C" void orc_rk4_step_f32s(const OrcEq *e, float *hi, float *lo, double t
   rhs_f32(e, hi, k, ts[0]);
   for (i64 i = 0; i < n; ++i) {
     D[i] = dt6 * (double)k[i];
-    y[i] = hi[i] + cy[0] * k[i];
+    y[i] = std::fmaf(cy[0], k[i], hi[i]);
   }
   for (int s = 1; s <= 2; ++s) {
     rhs_f32(e, y, k, ts[s]);
     for (i64 i = 0; i < n; ++i) {
-      D[i] = D[i] + dt3 * (double)k[i];
-      y[i] = hi[i] + cy[s] * k[i];
+      D[i] = std::fma(dt3, (double)k[i], D[i]);
+      y[i] = std::fmaf(cy[s], k[i], hi[i]);
     }
   }
   rhs_f32(e, y, k, ts[3]);
   for (i64 i = 0; i < n; ++i) {
-    const double s = ((double)hi[i] + (double)lo[i]) + (D[i] + dt6 * (double)k[i]);
+    const double s = ((double)hi[i] + (double)lo[i]) + std::fma(dt6, (double)k[i], D[i]);
     const float h = (float)s;
     hi[i] = h;
     lo[i] = (float)(s - (double)h);
