@@ -27,6 +27,7 @@ host on a bounded sample.
     python bench.py --workload adjoint                    # loss + gradient, N=128, 1024 steps
     python bench.py --workload variants --steps 10        # fixed-curvature / sqrt-potential steps
     python bench.py --workload slab                       # one GPU's slab of N=512 (64 planes + ghosts)
+    python bench.py --workload pacing --steps 20          # the f32s step at a list of paces
 """
 import argparse
 import json
@@ -133,11 +134,16 @@ def parse():
                     help="precision: the arithmetic of run A (f32s: float stencil, double increment; "
                          "f32c: FP64)")
     ap.add_argument("--save-dir", default="", help="ensemble: where to save the averaged density spectra (.npy)")
-    ap.add_argument("--workload", choices=("headline", "ensemble", "precision", "adjoint", "variants", "slab"),
+    ap.add_argument("--workload", choices=("headline", "ensemble", "precision", "adjoint", "variants", "slab",
+                                           "pacing"),
                     default="headline",
                     help="headline: N=512 FRW, FP32-comp vs FP64; ensemble: 96 realisations of N=64/96/128; "
                          "precision: run A (FP32-comp) vs run B (FP64) errors every 100 steps; "
-                         "adjoint: loss + gradient through 1024 steps at N=128")
+                         "adjoint: loss + gradient through 1024 steps at N=128; pacing: the f32s step at "
+                         "a list of paces")
+    ap.add_argument("--paces", default="0,100,110,115,120,125,130,135,140,150,160",
+                    help="pacing: ticks of the 100 MHz real-time counter per plane iteration (0: unpaced); "
+                         "the tuner's pick is timed as well")
     ap.add_argument("--report-every", type=int, default=100, help="precision: steps between error reports")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run 3 device-to-device copies of 1 GiB (cf_d2d), the known-size "
@@ -412,6 +418,61 @@ def adjoint_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def pacing_workload(args):
+    """The single-precision-stencil step (f32s) of the N=512 FRW workload at each pace of
+    --paces and at the tuner's pick, --steps timed steps per pace after 4 untimed ones,
+    two passes over the list. Paced and unpaced launches are different kernels
+    (k_rk4_tb_sa<LAM, true> / <LAM, false>), so a profiler separates them."""
+    import torch
+    import cfl_amd as cfl
+    N, L, dt, t0 = args.n, float(args.n), 0.05, 1.0
+    p = cfl.kg_params(N, L, 1.0, 0.0, cfl.radiation(1.0, 0.5, 1.0))
+    plan = cfl.SpectrumPlan(N)
+    x0 = plan.init_grf_state(2, cfl.CF_SPEC_GAUSS, (0.25,), L, 1.0)
+    del plan
+    settle(cfl, p, x0)
+    hi, lo = cfl.split_f64(x0)
+    del x0
+    bufs = [(hi, lo), tuple(torch.empty(2 * N ** 3, dtype=torch.float32, device="cuda") for _ in range(2))]
+    work = cfl.alloc_tb_work(N, cfl.CF_PREC_F32S)
+    t_tune = time.perf_counter()
+    tuned = cfl.rk4_tb_tune_pace(p, hi, lo, t0, dt, work, args.chunk)
+    tune_s = time.perf_counter() - t_tune
+    del work
+    paces = [int(q) for q in args.paces.split(",") if q] + [tuned]
+    stream = torch.cuda.current_stream()
+    rates = {}
+    for _ in range(2):
+        for pace in paces:
+            cfl.rk4_tb_set_pace(N, pace, args.chunk)
+            for k in range(4):
+                i, o = bufs[k & 1], bufs[(k + 1) & 1]
+                cfl.rk4_step_tb_f32s(p, i[0], i[1], o[0], o[1], t0 + k * dt, dt, args.chunk)
+            torch.cuda.synchronize()
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record(stream)
+            for k in range(args.steps):
+                i, o = bufs[k & 1], bufs[(k + 1) & 1]
+                cfl.rk4_step_tb_f32s(p, i[0], i[1], o[0], o[1], t0 + k * dt, dt, args.chunk)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            ms = ev0.elapsed_time(ev1) / args.steps
+            rates.setdefault(str(pace), []).append(N ** 3 / (ms * 1e-3) / 1e9)
+    cfl.rk4_tb_set_pace(N, tuned, args.chunk)
+    best = max(rates, key=lambda q: max(rates[q]))
+    line = {
+        "metric": "lattice site-steps per second of the f32s step by pace",
+        "value": max(rates[str(tuned)]), "unit": "GSite*RK4-step/s at the tuned pace", "higher_is_better": True,
+        "n_gpus": 1, "steps": args.steps, "dtype": "f32 stencil, f64 increment",
+        "data": "synthetic (Gaussian random field, seed 2)",
+        "config": {"workload": f"KG-FRW radiation era N={N}^3 L={L:g} m=1 dt=0.05 RK4, f32s step",
+                   "pace_unit": "ticks of the 100 MHz real-time counter per plane iteration"},
+        "tuned_pace": tuned, "tune_s": tune_s, "best_pace": int(best),
+        "rates": {q: [round(r, 2) for r in v] for q, v in rates.items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def slab_workload(args):
     """One GPU's share of a slab decomposition along a: the blocked step (FP32-comp, and
     FP64 beside it) on 64 of the 512 planes of the headline lattice (KG-FRW N=512, eight
@@ -560,6 +621,9 @@ def main():
         return
     if args.workload == "slab":
         slab_workload(args)
+        return
+    if args.workload == "pacing":
+        pacing_workload(args)
         return
     if args.workload == "precision":
         precision_workload(args)

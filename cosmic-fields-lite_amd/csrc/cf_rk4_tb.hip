@@ -488,8 +488,14 @@ cf_status cf_rk4_step_tb_f32s(const cf_kg_params *p, const float *hi_in, const f
 }
 
 // Pacing of the single-precision-stencil step: time the step of this lattice at a
-// range of paces and keep the fastest (the largest pace within 1% of the fastest: the
-// rate falls off a cliff just below the best pace and only slowly above it).
+// range of paces and keep one with margin. The rate falls off a cliff just below the
+// fastest pace (blocks that are not held back drift out of step) and declines slowly
+// above it, and where the cliff lies moves with the clock; a pace one or two ticks above
+// it measured fast while tuning and then ran 10-15% slow. So each pace P is scored by
+// the slowest of P - TB_PACE_MARGIN .. P, and the best score wins (at equal scores the
+// larger pace); unpaced wins if it is as fast.
+constexpr int TB_PACE_MARGIN = 4;
+
 cf_status cf_rk4_tb_tune_pace(const cf_kg_params *p, const float *hi, const float *lo, double t, double dt, int chunk,
                               void *work, cf_stream stream, int *pace_out) {
   CF_TRY(tb_check(p));
@@ -523,22 +529,33 @@ cf_status cf_rk4_tb_tune_pace(const cf_kg_params *p, const float *hi, const floa
     ms[pace] = el / 3.0f;
     return CF_OK;
   };
-  // coarse grid 0.64 .. 3.2 us per plane, then every tick around the coarse best
+  // coarse grid 0.64 .. 3.2 us per plane, then every tick from below the coarse best to
+  // the margin above it
   for (int pace = 0; pace <= 320 && rc == CF_OK; pace += (pace == 0 ? 64 : 8)) rc = time_pace(pace);
   int coarse = 0;
   for (const auto &kv : ms)
     if (kv.second < ms[coarse]) coarse = kv.first;
   if (coarse > 0)
-    for (int pace = coarse - 8; pace <= coarse + 8 && rc == CF_OK; ++pace) rc = time_pace(pace);
+    for (int pace = coarse - 8; pace <= coarse + 8 + TB_PACE_MARGIN && rc == CF_OK; ++pace) rc = time_pace(pace);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   CF_TRY(rc);
-  float best = ms[0];
-  for (const auto &kv : ms) best = kv.second < best ? kv.second : best;
-  int pick = 0;  // unpaced if that is within 1% of the fastest
-  if (ms[0] > best * 1.01f)
-    for (const auto &kv : ms)
-      if (kv.second <= best * 1.01f) pick = kv.first;  // ascending: the largest such pace
+  int pick = 0;
+  float pick_worst = ms[0];
+  for (const auto &kv : ms) {
+    if (kv.first - TB_PACE_MARGIN < 1) continue;
+    float worst = 0.0f;
+    bool full = true;
+    for (int q = kv.first - TB_PACE_MARGIN; q <= kv.first && full; ++q) {
+      const auto it = ms.find(q);
+      full = it != ms.end();
+      if (full) worst = it->second > worst ? it->second : worst;
+    }
+    if (full && worst <= pick_worst && (pick > 0 || worst < ms[0])) {
+      pick = kv.first;
+      pick_worst = worst;
+    }
+  }
   CF_TRY(tb_pace_store(p->N, g.A, pick));
   if (pace_out) *pace_out = pick;
   return CF_OK;

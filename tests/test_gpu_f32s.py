@@ -78,7 +78,7 @@ def test_f32s_tune_pace_keeps_state(cfl, O):
     work = cfl.alloc_tb_work(N, cfl.CF_PREC_F32S)
     pace = cfl.rk4_tb_tune_pace(p, hd, ld, 1.0, 0.05, work)
     print(f"tuned pace N={N}: {pace}", flush=True)
-    assert 0 <= pace <= 328 and cfl.rk4_tb_get_pace(N) == pace
+    assert 0 <= pace <= 332 and cfl.rk4_tb_get_pace(N) == pace
     assert np.array_equal(host(hd), hi) and np.array_equal(host(ld), lo)
     cfl.rk4_run_tb_f32s(p, hd, ld, 1.0, 0, 2, 0.05, work)  # with the tuned pace
     cfl.rk4_tb_set_pace(N, 0)
