@@ -131,8 +131,8 @@ def parse():
                     help="headline: kernels to time (tb_f32s, tb_f32c, tb_f64: one temporally blocked kernel "
                          "per step; f32c, f64: four fused stage kernels per step)")
     ap.add_argument("--run-a", choices=("f32s", "f32c"), default="f32s",
-                    help="precision: the arithmetic of run A (f32s: float stencil, double increment; "
-                         "f32c: FP64)")
+                    help="precision, ensemble: the arithmetic of the compensated storage (f32s: float "
+                         "stencil, double increment; f32c: FP64)")
     ap.add_argument("--save-dir", default="", help="ensemble: where to save the averaged density spectra (.npy)")
     ap.add_argument("--workload", choices=("headline", "ensemble", "precision", "adjoint", "variants", "slab",
                                            "pacing"),
@@ -213,11 +213,12 @@ def reference_integrator_timing(N, x, cores, steps=2):
 
 def ensemble_workload(args):
     """96 seeded realisations (seeds 100-131: N=64, 132-163: N=96, 164-195: N=128,
-    L=N), FRW Klein-Gordon with m=1 plus lambda phi^4/4 (lambda=0.1), FP32-comp,
-    P(k) ~ exp(-k^2/(2*0.5^2)), rms 0.5, RK4 dt=0.05, advanced by one persistent
-    kernel per step captured in a hipGraph, with the plan (grid, and per N the block
-    region shape and a-chunk) picked by the autotuner. Reports aggregate
-    GSite*step/s and the ensemble-averaged density spectrum of each N group."""
+    L=N), FRW Klein-Gordon with m=1 plus lambda phi^4/4 (lambda=0.1), compensated FP32
+    storage (--run-a: f32s, float forces with a double increment and paced blocks, or
+    f32c, FP64 arithmetic), P(k) ~ exp(-k^2/(2*0.5^2)), rms 0.5, RK4 dt=0.05, advanced
+    by one launch per step captured in a hipGraph, with the plan (grid, and per N the
+    block region shape and a-chunk; for f32s the pace) picked by the autotuner. Reports
+    aggregate GSite*step/s and the ensemble-averaged density spectrum of each N group."""
     import numpy as np
     import torch
     import cfl_amd as cfl
@@ -225,7 +226,8 @@ def ensemble_workload(args):
     seeds = list(range(100, 196))
     dt, t0 = 0.05, 1.0
     steps = args.steps
-    ens = cfl.Ensemble(Ns, [float(n) for n in Ns], 1.0, 0.1, cfl.radiation(1.0, 0.5, 1.0), cfl.CF_PREC_F32C)
+    prec = cfl.CF_PREC_F32S if args.run_a == "f32s" else cfl.CF_PREC_F32C
+    ens = cfl.Ensemble(Ns, [float(n) for n in Ns], 1.0, 0.1, cfl.radiation(1.0, 0.5, 1.0), prec)
     ens.init_grf(seeds, cfl.CF_SPEC_GAUSS, (0.5,), 0.5)
     tuned_grid = ens.tune(dt, reps=3)
     plan = {n: ens.get_plan(n)[0] for n in (64, 96, 128)}
@@ -260,12 +262,14 @@ def ensemble_workload(args):
             np.save(os.path.join(args.save_dir, f"ensemble_rho_spectrum_N{n}.npy"), avg)
         summary[str(n)] = {"bins": int(avg.size), "sum": float(avg.sum()), "first_bins": [float(v) for v in avg[1:5]]}
     line = {
-        "metric": "aggregate GSite*RK4-step/s, ensemble of 96 uneven-N realisations (FP32-comp, hipGraph)",
+        "metric": f"aggregate GSite*RK4-step/s, ensemble of 96 uneven-N realisations (FP32-comp {args.run_a}, "
+                  "hipGraph)",
         "value": rate, "unit": "GSite*RK4-step/s", "n_gpus": 1, "steps": steps, "warmup": 0,
         "ms_per_step": ms / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "storage": "f32 hi + f32 compensation", "data": "synthetic (Gaussian random fields, seeds 100-195)",
+        "dtype": "f32 stencil, f64 increment" if args.run_a == "f32s" else "f64",
+        "storage": "f32 hi + f32 compensation", "data": "synthetic (Gaussian random fields, seeds 100-195)",
         "config": {"workload": "ensemble 32xN=64 + 32xN=96 + 32xN=128, L=N, FRW KG m=1 lambda=0.1, dt=0.05",
-                   "sites": ens.sites, "tuned_grid": "flat (one block per unit)" if tuned_grid == cfl.CF_ENS_GRID_FLAT else tuned_grid, "tuned_chunks": plan, "tuned_regions": regions, "graph_wall_s": wall},
+                   "sites": ens.sites, "tuned_grid": "flat (one block per unit)" if tuned_grid == cfl.CF_ENS_GRID_FLAT else tuned_grid, "tuned_chunks": plan, "tuned_regions": regions, "tuned_pace": ens.get_pace(), "graph_wall_s": wall},
         "ensemble_avg_density_spectra": summary,
     }
     print(json.dumps(line), flush=True)

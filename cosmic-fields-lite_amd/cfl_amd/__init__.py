@@ -383,7 +383,7 @@ class Ensemble:
         n = 2 * N ** 3
         x, lo = self.state_ptrs(r)
         out = torch.empty(n, dtype=torch.float64, device="cuda")
-        if self.precision == CF_PREC_F32C:
+        if self.precision in (CF_PREC_F32C, CF_PREC_F32S):
             check(lib.cf_join_f64(C.c_void_p(x), C.c_void_p(lo), _ptr(out), n, _stream()))
         else:
             check(lib.cf_d2d(_ptr(out), C.c_void_p(x), n * 8, _stream()))
@@ -404,6 +404,12 @@ class Ensemble:
         r = C.c_int()
         check(lib.cf_ensemble_get_region(self.h, N, C.byref(r)))
         return r.value
+
+    def set_pace(self, pace):
+        check(lib.cf_ensemble_set_pace(self.h, pace))
+
+    def get_pace(self):
+        return lib.cf_ensemble_get_pace(self.h)
 
     def tune(self, dt, reps=3):
         g = C.c_int()

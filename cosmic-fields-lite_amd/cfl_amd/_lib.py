@@ -113,6 +113,8 @@ _SIGS = [
     ("cf_ensemble_set_region", st, [vp, i64, C.c_int]),
     ("cf_ensemble_get_region", st, [vp, i64, C.POINTER(C.c_int)]),
     ("cf_ensemble_tune", st, [vp, dbl, C.c_int, C.POINTER(C.c_int)]),
+    ("cf_ensemble_set_pace", st, [vp, C.c_int]),
+    ("cf_ensemble_get_pace", C.c_int, [vp]),
     ("cf_ensemble_run", st, [vp, dbl, i64, i64, dbl, C.c_int, vp]),
     ("cf_ensemble_density_spectrum", st, [vp, C.c_int, dbl, vp, vp]),
     ("cf_adjoint_work_bytes", C.c_size_t, [i64, i64, i64]),

@@ -78,9 +78,11 @@ __device__ __forceinline__ V wave_uniform(const V &v) {
 
 // One work unit of the step: its lattice's step scalars, then the temporally
 // blocked body over a region of RB x RC columns.
-template <class T, bool F32C, bool LAM, bool DEFER, bool LANE_NBR = true, int RB = TBR, int RC = TBR>
+template <class T, bool F32C, bool LAM, bool DEFER, bool LANE_NBR = true, int RB = TBR, int RC = TBR, bool SA = false,
+          bool PACE = false>
 __device__ __forceinline__ void ens_unit(TbLds<T> &lds, const EnsReal *__restrict__ reals, const EnsUnit &e,
-                                         const EnsStepCoef &sc, int parity, double c_m, double lam, double dt) {
+                                         const EnsStepCoef &sc, int parity, double c_m, double lam, double dt,
+                                         int pace = 0) {
   const EnsReal &R = reals[e.real];
   // Select this step's buffers first, then make each pointer uniform: indexing
   // x[parity] after the struct is in registers leaves the compiler treating the
@@ -109,6 +111,19 @@ __device__ __forceinline__ void ens_unit(TbLds<T> &lds, const EnsReal *__restric
   k.dt6 = dt / 6.0;
   k.dt3 = dt / 3.0;
   k = wave_uniform(k);  // the divisions above ran on the vector unit (no scalar FP64)
+  TbStepF kf{};  // single-precision stencil: the scalars rounded to float, as on the host
+  if constexpr (SA) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      kf.c_h[s] = (float)k.c_h[s];
+      kf.inv[s] = (float)k.inv[s];
+    }
+    kf.c_m = (float)k.c_m;
+    kf.lam = (float)k.lam;
+    kf.dh = (float)k.dh;
+    kf.dt = (float)k.dt;
+    kf = wave_uniform(kf);
+  }
   TbUnit u;
   u.N = N;
   u.b0 = e.b0;
@@ -118,20 +133,23 @@ __device__ __forceinline__ void ens_unit(TbLds<T> &lds, const EnsReal *__restric
   u.T = e.T;
   u.Tc = e.Tc;
   u.NA = N;
-  u.pace = 0;
-  tb_unit<T, F32C, LAM, DEFER, LANE_NBR, TB_KG, false, RB, RC>(lds, u, k, xin, lin, xout, lout);
+  u.pace = pace;
+  tb_unit<T, F32C, LAM, DEFER, LANE_NBR, TB_KG, false, RB, RC, SA, PACE>(lds, u, k, xin, lin, xout, lout, nullptr,
+                                                                       TbPsi{}, kf);
 }
 
 // A unit in its region shape (a wave-uniform branch; lane neighbours only in rows of
 // 32 columns).
-template <class T, bool F32C, bool LAM, bool DEFER, bool LANE_NBR>
+template <class T, bool F32C, bool LAM, bool DEFER, bool LANE_NBR, bool SA = false, bool PACE = false>
 __device__ __forceinline__ void ens_unit_any(TbLds<T> &lds, const EnsReal *__restrict__ reals, const EnsUnit &unit,
-                                             const EnsStepCoef &sc, int parity, double c_m, double lam, double dt) {
+                                             const EnsStepCoef &sc, int parity, double c_m, double lam, double dt,
+                                             int pace = 0) {
   const EnsUnit e = wave_uniform(unit);
   if (e.region == CF_ENS_REGION_24X40)
-    ens_unit<T, F32C, LAM, DEFER, false, ENS_R1_B, ENS_R1_C>(lds, reals, e, sc, parity, c_m, lam, dt);
+    ens_unit<T, F32C, LAM, DEFER, false, ENS_R1_B, ENS_R1_C, SA, PACE>(lds, reals, e, sc, parity, c_m, lam, dt,
+                                                                        pace);
   else
-    ens_unit<T, F32C, LAM, DEFER, LANE_NBR>(lds, reals, e, sc, parity, c_m, lam, dt);
+    ens_unit<T, F32C, LAM, DEFER, LANE_NBR, TBR, TBR, SA, PACE>(lds, reals, e, sc, parity, c_m, lam, dt, pace);
 }
 
 // Persistent form: a fixed grid; block b runs units unit_off[b] .. unit_off[b+1]-1
@@ -174,12 +192,45 @@ __global__ __launch_bounds__(1024) void k_ens_tb_flat(const EnsReal *__restrict_
   ens_unit_any<T, F32C, LAM, true, !F32C>(lds, reals, units[u], steps[step], parity, c_m, lam, dt);
 }
 
+// The same two forms for the compensated storage with the single-precision stencil
+// (CF_PREC_F32S), paced (PACE: every unit's plane iterations follow pace ticks of the
+// real-time counter from the unit's start; cf_tb_body.hpp). Occupancy is one 1024-thread
+// block per CU regardless, so the compiler may use the 128 VGPRs that allows.
+template <bool LAM, bool PACE>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_ens_tb_sa(
+    const EnsReal *__restrict__ reals, const int32_t *__restrict__ unit_off, const EnsUnit *__restrict__ units,
+    const EnsStepCoef *__restrict__ steps, int64_t step, int parity, double c_m, double lam, double dt, int pace) {
+  __shared__ TbLds<float> lds;
+  const EnsStepCoef sc = steps[step];
+  const int beg = unit_off[blockIdx.x], end = unit_off[blockIdx.x + 1];
+  for (int i = beg; i < end; ++i) {
+    ens_unit_any<float, true, LAM, false, false, true, PACE>(lds, reals, units[i], sc, parity, c_m, lam, dt, pace);
+    __syncthreads();
+  }
+}
+
+template <bool LAM, bool PACE>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_ens_tb_flat_sa(
+    const EnsReal *__restrict__ reals, const EnsUnit *__restrict__ units, const EnsStepCoef *__restrict__ steps,
+    int64_t step, int parity, double c_m, double lam, double dt, int pace) {
+  __shared__ TbLds<float> lds;
+  int u = blockIdx.x;
+  const int total = gridDim.x;
+  if (total % (8 * ENS_XCD_GROUP) == 0) {
+    const int x = u & 7, i = u >> 3;
+    u = ((i / ENS_XCD_GROUP) * 8 + x) * ENS_XCD_GROUP + (i % ENS_XCD_GROUP);
+  }
+  ens_unit_any<float, true, LAM, true, false, true, PACE>(lds, reals, units[u], steps[step], parity, c_m, lam, dt,
+                                                          pace);
+}
+
 }  // namespace cf
 
 using namespace cf;
 
 struct cf_ensemble {
   int n = 0, prec = CF_PREC_F64;
+  int pace = 0;  // CF_PREC_F32S: ticks per plane iteration (0: unpaced)
   double m = 0, lambda = 0;
   cf_cosmology cosmo{1, 0, 0, 1};
   std::vector<EnsReal> reals;
@@ -332,7 +383,37 @@ static cf_status ens_fill_steps(cf_ensemble *e, double t0, int64_t k0, int64_t n
   return CF_OK;
 }
 
+static cf_status ens_launch_sa(cf_ensemble *e, int64_t step, int parity, double dt, hipStream_t st) {
+  const double c_m = e->m * e->m;
+  const bool lam = e->lambda != 0.0, paced = e->pace > 0;
+  const dim3 blk(1024);
+  if (e->grid == CF_ENS_GRID_FLAT) {
+    const dim3 grd(e->n_units);
+#define CF_ENS_FLAT_SA(L, P)                                                                                        \
+  hipLaunchKernelGGL((k_ens_tb_flat_sa<L, P>), grd, blk, 0, st, e->d_reals, e->d_units, e->d_steps, step, parity, \
+                     c_m, e->lambda, dt, e->pace)
+    if (lam && paced) CF_ENS_FLAT_SA(true, true);
+    else if (lam) CF_ENS_FLAT_SA(true, false);
+    else if (paced) CF_ENS_FLAT_SA(false, true);
+    else CF_ENS_FLAT_SA(false, false);
+#undef CF_ENS_FLAT_SA
+  } else {
+    const dim3 grd(e->grid);
+#define CF_ENS_SA(L, P)                                                                                            \
+  hipLaunchKernelGGL((k_ens_tb_sa<L, P>), grd, blk, 0, st, e->d_reals, e->d_unit_off, e->d_units, e->d_steps, step, \
+                     parity, c_m, e->lambda, dt, e->pace)
+    if (lam && paced) CF_ENS_SA(true, true);
+    else if (lam) CF_ENS_SA(true, false);
+    else if (paced) CF_ENS_SA(false, true);
+    else CF_ENS_SA(false, false);
+#undef CF_ENS_SA
+  }
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
 static cf_status ens_launch(cf_ensemble *e, int64_t step, int parity, double dt, hipStream_t st) {
+  if (e->prec == CF_PREC_F32S) return ens_launch_sa(e, step, parity, dt, st);
   dim3 grd(e->grid), blk(1024);
   const double c_m = e->m * e->m;
   const bool lam = e->lambda != 0.0;
@@ -410,9 +491,61 @@ static cf_status ens_time(cf_ensemble *e, int reps, double dt, float *ms) {
   return rc;
 }
 
+// CF_PREC_F32S: the pace of the units' plane iterations, as the single-lattice tuner
+// (cf_rk4_tb_tune_pace) picks it: a coarse grid of paces, then tick by tick around the
+// best, each pace P scored by the slowest of P-4..P (a margin above the cliff below the
+// fastest pace); unpaced if that is as fast.
+static cf_status ens_tune_pace(cf_ensemble *e, int reps, double dt) {
+  std::map<int, float> ms;
+  auto time_pace = [&](int pace) -> cf_status {
+    if (ms.count(pace)) return CF_OK;
+    e->pace = pace;
+    float t = 0.0f;
+    CF_TRY(ens_time(e, reps, dt, &t));
+    ms[pace] = t;
+    return CF_OK;
+  };
+  for (int pace = 0; pace <= 320; pace += (pace == 0 ? 64 : 8)) CF_TRY(time_pace(pace));
+  int coarse = 0;
+  for (const auto &kv : ms)
+    if (kv.second < ms[coarse]) coarse = kv.first;
+  constexpr int MARGIN = 4;
+  if (coarse > 0)
+    for (int pace = coarse - 8; pace <= coarse + 8 + MARGIN; ++pace) CF_TRY(time_pace(pace));
+  int pick = 0;
+  float pick_worst = ms[0];
+  for (const auto &kv : ms) {
+    if (kv.first - MARGIN < 1) continue;
+    float worst = 0.0f;
+    bool full = true;
+    for (int q = kv.first - MARGIN; q <= kv.first && full; ++q) {
+      const auto it = ms.find(q);
+      full = it != ms.end();
+      if (full) worst = it->second > worst ? it->second : worst;
+    }
+    if (full && worst <= pick_worst && (pick > 0 || worst < ms[0])) {
+      pick = kv.first;
+      pick_worst = worst;
+    }
+  }
+  e->pace = pick;
+  return CF_OK;
+}
+
 }  // namespace cf
 
 extern "C" {
+
+cf_status cf_ensemble_set_pace(cf_ensemble *e, int pace) {
+  CF_REQUIRE(e, "cf_ensemble_set_pace: null ensemble");
+  CF_REQUIRE(pace >= 0 && pace <= 100000, "cf_ensemble_set_pace: pace %d outside 0..100000", pace);
+  CF_REQUIRE(pace == 0 || e->prec == CF_PREC_F32S, "cf_ensemble_set_pace: only CF_PREC_F32S ensembles are paced");
+  ens_drop_graph(e);  // a captured graph holds the old pace as a kernel argument
+  e->pace = pace;
+  return CF_OK;
+}
+
+int cf_ensemble_get_pace(const cf_ensemble *e) { return e ? e->pace : -1; }
 
 cf_status cf_ensemble_destroy(cf_ensemble *e) {
   if (!e) return CF_OK;
@@ -434,7 +567,8 @@ cf_status cf_ensemble_destroy(cf_ensemble *e) {
 cf_status cf_ensemble_create(int n, const int64_t *N, const double *L, double m, double lambda,
                              const cf_cosmology *cosmo, int precision, cf_ensemble **out) {
   CF_REQUIRE(out && N && L && cosmo && n > 0, "cf_ensemble_create: bad arguments");
-  CF_REQUIRE(precision == CF_PREC_F64 || precision == CF_PREC_F32C, "cf_ensemble_create: unknown precision");
+  CF_REQUIRE(precision == CF_PREC_F64 || precision == CF_PREC_F32C || precision == CF_PREC_F32S,
+             "cf_ensemble_create: unknown precision");
   *out = nullptr;
   for (int r = 0; r < n; ++r)
     CF_REQUIRE(valid_N(N[r]) && N[r] <= 1024 && L[r] > 0, "cf_ensemble_create: realisation %d: bad N or L", r);
@@ -463,14 +597,14 @@ cf_status cf_ensemble_create(int n, const int64_t *N, const double *L, double m,
     const size_t cnt = 2 * (size_t)N[r] * N[r] * N[r];
     for (int q = 0; q < 2; ++q) {
       void *p = nullptr;
-      const size_t bytes = cnt * (precision == CF_PREC_F32C ? sizeof(float) : sizeof(double));
+      const size_t bytes = cnt * (precision != CF_PREC_F64 ? sizeof(float) : sizeof(double));
       if (hipMalloc(&p, bytes) != hipSuccess) {
         set_error("cf_ensemble_create: out of device memory");
         return fail(CF_E_NOMEM);
       }
       e->allocs.push_back(p);
       R.x[q] = p;
-      if (precision == CF_PREC_F32C) {
+      if (precision != CF_PREC_F64) {  // compensated storage: hi in x, lo here
         void *l = nullptr;
         if (hipMalloc(&l, cnt * sizeof(float)) != hipSuccess) {
           set_error("cf_ensemble_create: out of device memory");
@@ -506,7 +640,7 @@ int64_t cf_ensemble_sites(const cf_ensemble *e) {
 cf_status cf_ensemble_state(cf_ensemble *e, int r, void **x_dev, float **lo_dev) {
   CF_REQUIRE(e && r >= 0 && r < e->n && x_dev, "cf_ensemble_state: bad arguments");
   *x_dev = e->reals[r].x[e->parity];
-  if (lo_dev) *lo_dev = e->prec == CF_PREC_F32C ? e->reals[r].lo[e->parity] : nullptr;
+  if (lo_dev) *lo_dev = e->prec != CF_PREC_F64 ? e->reals[r].lo[e->parity] : nullptr;
   return CF_OK;
 }
 
@@ -556,7 +690,7 @@ cf_status cf_ensemble_init_grf(cf_ensemble *e, const uint32_t *seeds, int kind, 
     const int64_t cnt = 2 * R.N * R.N * R.N;
     double *dst = e->prec == CF_PREC_F64 ? (double *)R.x[e->parity] : e->scratch;
     CF_TRY(cf_init_grf_state(pl, seeds[r], kind, params, R.L, rms, dst, stream));
-    if (e->prec == CF_PREC_F32C)
+    if (e->prec != CF_PREC_F64)
       CF_TRY(cf_split_f64(e->scratch, (float *)R.x[e->parity], R.lo[e->parity], cnt, stream));
   }
   CF_HIP(hipStreamSynchronize(S(stream)));
@@ -584,8 +718,11 @@ cf_status cf_ensemble_tune(cf_ensemble *e, double dt, int reps, int *grid_out) {
   // per lattice size, every (region shape, chunk) pair, the other sizes' plans held;
   // a second sweep revisits each size once the others have their tuned plans. Chunks
   // go longest first and a candidate must win by more than 0.5%: within the timing
-  // noise the longer chunk (fewer warm-up planes) is kept.
+  // noise the longer chunk (fewer warm-up planes) is kept. A paced ensemble
+  // (CF_PREC_F32S) tunes its pace between the two sweeps, so the second one times the
+  // plans paced.
   for (int sweep = 0; sweep < 2; ++sweep) {
+    if (sweep == 1 && e->prec == CF_PREC_F32S) CF_TRY(ens_tune_pace(e, reps, dt));
     for (auto &kv : e->chunk) {
       const int64_t N = kv.first;
       int best_a = kv.second, best_r = e->region[N];
@@ -661,7 +798,7 @@ cf_status cf_ensemble_density_spectrum(cf_ensemble *e, int r, double t, double *
   if (!pl) CF_TRY(cf_spectrum_plan_create(R.N, &pl));
   const int64_t n3 = R.N * R.N * R.N;
   const double *x = (const double *)R.x[e->parity];
-  if (e->prec == CF_PREC_F32C) {
+  if (e->prec != CF_PREC_F64) {
     CF_TRY(cf_join_f64((const float *)R.x[e->parity], R.lo[e->parity], e->scratch, 2 * n3, stream));
     x = e->scratch;
   }

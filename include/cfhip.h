@@ -226,13 +226,15 @@ cf_status cf_join_f64(const float *hi_dev, const float *lo_dev, double *x_dev, i
  * the reference runs one lattice per process, src/main.cpp:82-196). One persistent
  * kernel per step walks all realisations' work units; a run can be captured once
  * into a hipGraph and replayed. Each realisation's result equals a single-lattice
- * run of the same steps bit for bit. All realisations share m, lambda, cosmology. */
+ * run of the same steps bit for bit (cf_rk4_run_tb_f64 / _f32c / _f32s for precision
+ * CF_PREC_F64 / F32C / F32S). All realisations share m, lambda, cosmology. */
 typedef struct cf_ensemble cf_ensemble;
 cf_status cf_ensemble_create(int n, const int64_t *N, const double *L, double m, double lambda,
                              const cf_cosmology *cosmo, int precision, cf_ensemble **e);
 cf_status cf_ensemble_destroy(cf_ensemble *e);
 int64_t cf_ensemble_sites(const cf_ensemble *e); /* sum of N^3 */
-/* Current state of realisation r: x (FP64) or hi (F32C) and lo (F32C, else NULL).
+/* Current state of realisation r: x (FP64) or hi (F32C, F32S) and lo (F32C, F32S, else
+ * NULL).
  * Valid until the next run (states ping-pong). */
 cf_status cf_ensemble_state(cf_ensemble *e, int r, void **x_dev, float **lo_dev);
 /* Initial conditions: realisation r gets the GRF of seeds[r] (cf_init_grf_state). */
@@ -255,8 +257,15 @@ cf_status cf_ensemble_set_region(cf_ensemble *e, int64_t N, int region);
 cf_status cf_ensemble_get_region(const cf_ensemble *e, int64_t N, int *region);
 /* Autotuner: times candidate grids (persistent and flat), then per N candidate
  * region shapes and chunks, on the device (a step reads one buffer and writes the
- * other, so the state is untouched), and keeps the fastest plan. Blocking. */
+ * other, so the state is untouched), and keeps the fastest plan; a CF_PREC_F32S
+ * ensemble also tunes its pace (as cf_rk4_tb_tune_pace) before revisiting the plans.
+ * Blocking. */
 cf_status cf_ensemble_tune(cf_ensemble *e, double dt, int reps, int *grid_out);
+/* Pace of a CF_PREC_F32S ensemble's units (ticks of the 100 MHz real-time counter per
+ * plane iteration, 0: unpaced; see cf_rk4_tb_tune_pace). Changes timing only, never
+ * results; other precisions accept only 0. cf_ensemble_get_pace returns -1 for null. */
+cf_status cf_ensemble_set_pace(cf_ensemble *e, int pace);
+int cf_ensemble_get_pace(const cf_ensemble *e);
 /* n_steps RK4 steps at t_k = t0 + (k0 + k) dt; use_graph: capture (or reuse) a hipGraph.
  * Asynchronous: ordered after earlier work on `stream`, and later work on `stream`
  * waits for it. Consecutive runs (segments of one evolution) need no host

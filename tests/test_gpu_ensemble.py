@@ -3,7 +3,8 @@
 Every realisation must equal a single-lattice run of the same steps bit for bit
 (so the output does not depend on the batch composition), graph replays must
 equal the eager run (replayed several times with eager work in between), and
-the autotuner's plan must not change any bit.
+the autotuner's plan must not change any bit. Precisions: FP64 (0), FP32-comp (1)
+and the compensated storage with float forces (2, CF_PREC_F32S, paced or not).
 """
 import ctypes as C
 
@@ -20,11 +21,11 @@ NS = [16, 24, 32, 16, 40, 24]
 
 
 def raw_state(cfl, ens, r):
-    """hi, lo (F32C) or x (F64) of realisation r as fresh torch tensors."""
+    """hi, lo (F32C, F32S) or x (F64) of realisation r as fresh torch tensors."""
     N = ens.Ns[r]
     n = 2 * N ** 3
     x, lo = ens.state_ptrs(r)
-    if ens.precision == cfl.CF_PREC_F32C:
+    if ens.precision != cfl.CF_PREC_F64:
         hi_t = torch.empty(n, dtype=torch.float32, device="cuda")
         lo_t = torch.empty_like(hi_t)
         cfl.check(cfl.lib.cf_d2d(cfl._ptr(hi_t), C.c_void_p(x), n * 4, cfl._stream()))
@@ -45,9 +46,12 @@ def check_single_lattice_runs(cfl, ens, prec, init, t0, k0, steps, dt, lam=0.1):
     """Every realisation of `ens` equals the single-lattice run from its initial state."""
     for r, N in enumerate(ens.Ns):
         p = cfl.kg_params(N, float(N), 1.0, lam, cfl.radiation(**FRW))
-        if prec == cfl.CF_PREC_F32C:
+        if prec != cfl.CF_PREC_F64:
             hi, lo = init[r]
-            cfl.rk4_run_f32c(p, hi, lo, t0, k0, steps, dt, cfl.alloc_work(N, prec))
+            if prec == cfl.CF_PREC_F32C:
+                cfl.rk4_run_f32c(p, hi, lo, t0, k0, steps, dt, cfl.alloc_work(N, prec))
+            else:
+                cfl.rk4_run_tb_f32s(p, hi, lo, t0, k0, steps, dt, cfl.alloc_tb_work(N, prec))
             ghi, glo = raw_state(cfl, ens, r)
             assert_bitwise(host(ghi), host(hi), f"hi r={r}")
             assert_bitwise(host(glo), host(lo), f"lo r={r}")
@@ -58,7 +62,7 @@ def check_single_lattice_runs(cfl, ens, prec, init, t0, k0, steps, dt, lam=0.1):
             assert_bitwise(host(gx), host(x), f"x r={r}")
 
 
-@pytest.mark.parametrize("prec,grid", [(0, 0), (1, 0), (0, -1), (1, -1)])
+@pytest.mark.parametrize("prec,grid", [(0, 0), (1, 0), (0, -1), (1, -1), (2, 0), (2, -1)])
 def test_ensemble_equals_single_lattice_runs(cfl, prec, grid):
     """Persistent (default grid) and flat (grid = CF_ENS_GRID_FLAT) launches."""
     print(f"case ensemble prec={prec} grid={grid}", flush=True)
@@ -73,7 +77,7 @@ def test_ensemble_equals_single_lattice_runs(cfl, prec, grid):
     ens.close()
 
 
-@pytest.mark.parametrize("prec,grid", [(0, 0), (1, 0), (0, -1), (1, -1)])
+@pytest.mark.parametrize("prec,grid", [(0, 0), (1, 0), (0, -1), (1, -1), (2, 0), (2, -1)])
 def test_region_shapes_keep_bits(cfl, prec, grid):
     """Blocks over 24 x 40 regions (16 x 32 tiles: one partial tile per side at N=24
     and N=40, a single tile at N=16) mixed with 32 x 32 ones in the same launches
@@ -94,7 +98,7 @@ def test_region_shapes_keep_bits(cfl, prec, grid):
     ens.close()
 
 
-@pytest.mark.parametrize("prec,grid", [(0, 0), (1, 0), (0, -1), (1, -1)])
+@pytest.mark.parametrize("prec,grid", [(0, 0), (1, 0), (0, -1), (1, -1), (2, 0), (2, -1)])
 def test_ensemble_without_lambda(cfl, prec, grid):
     """lambda = 0 selects the kernels without the phi^3 term; both region shapes."""
     print(f"case ensemble lambda=0 prec={prec} grid={grid}", flush=True)
@@ -121,7 +125,7 @@ def test_ensemble_oracle_small(cfl, O):
     ens.close()
 
 
-@pytest.mark.parametrize("prec,grid", [(0, 0), (1, 0), (1, -1)])
+@pytest.mark.parametrize("prec,grid", [(0, 0), (1, 0), (1, -1), (2, -1)])
 def test_graph_replays_equal_eager(cfl, prec, grid):
     print(f"case graph prec={prec} grid={grid}", flush=True)
     seeds = list(range(100, 100 + len(NS)))
@@ -142,8 +146,10 @@ def test_graph_replays_equal_eager(cfl, prec, grid):
     ens.close()
 
 
-def test_autotuner_keeps_bits(cfl):
-    ens = make(cfl, 1)
+@pytest.mark.parametrize("prec", [1, 2])
+def test_autotuner_keeps_bits(cfl, prec):
+    print(f"case ensemble tune prec={prec}", flush=True)
+    ens = make(cfl, prec)
     x0 = [raw_state(cfl, ens, r) for r in range(ens.n)]
     g = ens.tune(0.05, reps=2)
     assert g > 0 or g == cfl.CF_ENS_GRID_FLAT
@@ -151,10 +157,11 @@ def test_autotuner_keeps_bits(cfl):
     for r in range(ens.n):
         for a, b in zip(raw_state(cfl, ens, r), x0[r]):
             assert torch.equal(a, b)
+    print(f"tuned grid {g} pace {ens.get_pace()}", flush=True)
     ens.run(1.0, 0, 2, 0.05)
     after_tuned = [host(ens.state(r)) for r in range(ens.n)]
     ens.close()
-    ens2 = make(cfl, 1)
+    ens2 = make(cfl, prec)
     ens2.set_plan(0, 0, 97)  # an arbitrary different plan
     ens2.run(1.0, 0, 2, 0.05)
     for r in range(ens2.n):
@@ -236,3 +243,27 @@ def test_graph_not_replayed_after_eager_run(cfl):
             assert_bitwise(host(x), host(y), f"graph after eager r={r}")
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("grid", [0, -1])
+def test_paced_f32s_ensemble_keeps_bits(cfl, grid):
+    """A paced single-precision-stencil ensemble (both launch forms, graph replay)
+    gives every realisation the bits of its single-lattice run; pacing is refused
+    for the other precisions."""
+    print(f"case ensemble f32s paced grid={grid}", flush=True)
+    ens = make(cfl, cfl.CF_PREC_F32S)
+    if grid:
+        ens.set_plan(0, 0, grid)
+    ens.set_region(40, cfl.CF_ENS_REGION_24X40)
+    ens.set_pace(180)
+    assert ens.get_pace() == 180
+    t0, k0, steps, dt = 1.0, 1, 3, 0.05
+    init = [raw_state(cfl, ens, r) for r in range(ens.n)]
+    ens.run(t0, k0, steps, dt, use_graph=True)
+    check_single_lattice_runs(cfl, ens, cfl.CF_PREC_F32S, init, t0, k0, steps, dt)
+    ens.close()
+    other = make(cfl, cfl.CF_PREC_F32C)
+    with pytest.raises(cfl.CfError):
+        other.set_pace(100)
+    other.set_pace(0)
+    other.close()
