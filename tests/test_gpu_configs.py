@@ -8,7 +8,8 @@
   and the states within 1e-6 (relative L2). Run A in both compensated forms: FP64
   arithmetic (f32c) and float stencil arithmetic with a double increment (f32s).
 * The ensemble: 96 realisations, 32 each at N=64, 96, 128 (L=N, seeds 100-195 in
-  that order), FRW Klein-Gordon with m=1 and lambda=0.1, FP32-comp, P(k) ~
+  that order), FRW Klein-Gordon with m=1 and lambda=0.1, FP32-comp (both arithmetic
+  forms), P(k) ~
   exp(-k^2/(2*0.5^2)), rms 0.5, dt=0.05, the autotuned plan and graph replay, for 5
   steps. Every realisation must equal its single-lattice run bit for bit, one per N
   must equal the oracle's compensated step bit for bit, and the per-N average of
@@ -82,12 +83,15 @@ def test_precision_run_a_vs_run_b_n512(cfl, mode):
     plan.close()
 
 
-def test_ensemble_96_uneven_n(cfl, O):
+@pytest.mark.parametrize("mode", ["f32c", "f32s"])
+def test_ensemble_96_uneven_n(cfl, O, mode):
     Ns = [64] * 32 + [96] * 32 + [128] * 32
     seeds = list(range(100, 196))
     dt, t0, steps = 0.05, 1.0, 5
-    print("case ensemble 96 realisations, 5 steps", flush=True)
-    ens = cfl.Ensemble(Ns, [float(n) for n in Ns], 1.0, 0.1, cfl.radiation(**FRW), cfl.CF_PREC_F32C)
+    print(f"case ensemble 96 realisations {mode}, 5 steps", flush=True)
+    prec = cfl.CF_PREC_F32C if mode == "f32c" else cfl.CF_PREC_F32S
+    run_single = cfl.rk4_run_tb_f32c if mode == "f32c" else cfl.rk4_run_tb_f32s
+    ens = cfl.Ensemble(Ns, [float(n) for n in Ns], 1.0, 0.1, cfl.radiation(**FRW), prec)
     ens.init_grf(seeds, cfl.CF_SPEC_GAUSS, (0.5,), 0.5)
 
     def raw(r):
@@ -105,20 +109,21 @@ def test_ensemble_96_uneven_n(cfl, O):
     torch.cuda.synchronize()
 
     # every realisation against its single-lattice run (the temporally blocked step)
-    works = {n: cfl.alloc_tb_work(n, cfl.CF_PREC_F32C) for n in (64, 96, 128)}
+    works = {n: cfl.alloc_tb_work(n, prec) for n in (64, 96, 128)}
     for r, n in enumerate(Ns):
         hi, lo = init[r]
         if r % 32 == 0:
             hi0, lo0 = host(hi).copy(), host(lo).copy()
         p = cfl.kg_params(n, float(n), 1.0, 0.1, cfl.radiation(**FRW))
-        cfl.rk4_run_tb_f32c(p, hi, lo, t0, 0, steps, dt, works[n])
+        run_single(p, hi, lo, t0, 0, steps, dt, works[n])
         ghi, glo = raw(r)
         assert_bitwise(host(ghi), host(hi), f"hi r={r} N={n}")
         assert_bitwise(host(glo), host(lo), f"lo r={r} N={n}")
         if r % 32 == 0:  # the first realisation of each N against the oracle
             eq = O.Equation(n, float(n), 1.0, 0.1, O.Cosmo.radiation(**FRW))
+            oracle_step = eq.rk4_step_f32c if mode == "f32c" else eq.rk4_step_f32s
             for k in range(steps):
-                eq.rk4_step_f32c(hi0, lo0, t0 + k * dt, dt)
+                oracle_step(hi0, lo0, t0 + k * dt, dt)
             assert_bitwise(host(ghi), hi0, f"hi r={r} N={n} vs oracle")
             assert_bitwise(host(glo), lo0, f"lo r={r} N={n} vs oracle")
     del works, init
