@@ -14,7 +14,9 @@
 // hip_runge_kutta4_f32c steps a compensated_state: fields stored as float
 // hi + lo, stage inputs and the RK4 increment stored as float, all arithmetic
 // in double (the temporally blocked cf_rk4_step_tb_f32c, same bits as
-// cf_rk4_step_f32c), with the same buffer swap.
+// cf_rk4_step_f32c), with the same buffer swap. hip_runge_kutta4_f32s steps the
+// same state with float forces and a double increment (cf_rk4_step_tb_f32s; tune
+// its pace first with cf_rk4_tb_tune_pace for speed, the bits do not depend on it).
 #pragma once
 
 #if __has_include(<boost/numeric/odeint/stepper/stepper_categories.hpp>)
@@ -80,7 +82,8 @@ inline void to_f64(const compensated_state &s, hip_vector<double> &x) {
   check(cf_join_f64(s.hi.data(), s.lo.data(), x.data(), (int64_t)s.size(), default_stream()), "to_f64");
 }
 
-class hip_runge_kutta4_f32c {
+template <bool FLOAT_FORCES>
+class hip_runge_kutta4_compensated {
  public:
   typedef compensated_state state_type;
   typedef double value_type;
@@ -99,9 +102,13 @@ class hip_runge_kutta4_f32c {
       o.hi.resize(x.size());
       o.lo.resize(x.size());
     }
-    // the temporally blocked kernel (same bits as cf_rk4_step_f32c), then a swap
-    check(cf_rk4_step_tb_f32c(&p, x.hi.data(), x.lo.data(), o.hi.data(), o.lo.data(), t, dt, 0, default_stream()),
-          "hip_runge_kutta4_f32c");
+    // the temporally blocked kernel, then a swap
+    if constexpr (FLOAT_FORCES)
+      check(cf_rk4_step_tb_f32s(&p, x.hi.data(), x.lo.data(), o.hi.data(), o.lo.data(), t, dt, 0, default_stream()),
+            "hip_runge_kutta4_f32s");
+    else
+      check(cf_rk4_step_tb_f32c(&p, x.hi.data(), x.lo.data(), o.hi.data(), o.lo.data(), t, dt, 0, default_stream()),
+            "hip_runge_kutta4_f32c");
     x.hi.swap(o.hi);
     x.lo.swap(o.lo);
   }
@@ -109,5 +116,7 @@ class hip_runge_kutta4_f32c {
  private:
   std::shared_ptr<compensated_state> next_ = std::make_shared<compensated_state>();
 };
+typedef hip_runge_kutta4_compensated<false> hip_runge_kutta4_f32c;
+typedef hip_runge_kutta4_compensated<true> hip_runge_kutta4_f32s;
 
 }  // namespace cfl

@@ -7,6 +7,7 @@
 //   stepper:  classic (odeint_lite::runge_kutta4_classic over hip_operations)
 //             fused   (cfl::hip_runge_kutta4: fused stage kernels)
 //             f32c    (cfl::hip_runge_kutta4_f32c: FP32-compensated storage)
+//             f32s    (cfl::hip_runge_kutta4_f32s: the same storage, float forces, tuned pace)
 //   equation: kg | frw | lambda | sqrt (sqrt potential; the lambda argument is f_a)
 // Writes state0.dat (initial state), the observer's spectra (s-bins and N/2 radial bins)
 // and density slices, t_list.dat
@@ -52,15 +53,28 @@ static int run(const std::string &dir, const std::string &stepper, const RunPara
   } else if constexpr (!requires { eqn.params(); }) {
     std::fprintf(stderr, "stepper %s is not available for this equation\n", stepper.c_str());
     return 2;
-  } else if (stepper == "f32c") {
+  } else if (stepper == "f32c" || stepper == "f32s") {
     // Compensated state; the observer sees the joined double state.
     cfl::compensated_state xs = cfl::to_compensated(workspace.state);
-    cfl::hip_runge_kutta4_f32c st;
     auto obs = [&](const cfl::compensated_state &s, double t) {
       cfl::to_f64(s, workspace.state);
       observer(workspace.state, t);
     };
-    steps = cfl::odeint_lite::integrate_const(st, eqn, xs, param.t_start, param.t_end, param.delta_t, obs);
+    if (stepper == "f32c") {
+      steps = cfl::odeint_lite::integrate_const(cfl::hip_runge_kutta4_f32c(), eqn, xs, param.t_start, param.t_end,
+                                                param.delta_t, obs);
+    } else {
+      // the pace of the float-force step for this lattice (timing only; the state is
+      // left as it is)
+      const cf_kg_params kp = eqn.params();
+      cfl::hip_vector<unsigned char> work(cf_rk4_tb_work_bytes(kp.N, CF_PREC_F32S));
+      int pace = 0;
+      cfl::check(cf_rk4_tb_tune_pace(&kp, xs.hi.data(), xs.lo.data(), param.t_start, param.delta_t, 0, work.data(),
+                                     cfl::default_stream(), &pace),
+                 "cf_rk4_tb_tune_pace");
+      steps = cfl::odeint_lite::integrate_const(cfl::hip_runge_kutta4_f32s(), eqn, xs, param.t_start, param.t_end,
+                                                param.delta_t, obs);
+    }
     cfl::to_f64(xs, workspace.state);
   } else {
     std::fprintf(stderr, "unknown stepper %s\n", stepper.c_str());

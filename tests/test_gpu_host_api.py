@@ -39,7 +39,7 @@ def run_kg(tmp_path, stepper, eq, N=32, L=40.0, m=1.0, lam=0.0, t0=1.0, t1=3.0, 
                      eq=eq)
 
 
-def oracle_run(O, out, p, f32c=False):
+def oracle_run(O, out, p, f32c=False, f32s=False):
     N = p["N"]
     x = np.fromfile(out + "state0.dat")
     cosmo = O.Cosmo.radiation(p["a1"], p["H1"], p["t1c"]) if p["H1"] != 0 else O.Cosmo.minkowski()
@@ -58,15 +58,16 @@ def oracle_run(O, out, p, f32c=False):
         return x, obs
     eq = O.Equation(N, p["L"], p["m"], p["lam"], cosmo)
     obs = O.ConstIntervalObserver(eq, p["t0"], p["t1"], p["interval"], save_density=True)
-    if f32c:
+    if f32c or f32s:
         # the driver splits the state once and observes hi + lo from then on,
         # including at t_start
         hi = x.astype(np.float32)
         lo = (x - hi.astype(np.float64)).astype(np.float32)
         x[:] = hi.astype(np.float64) + lo.astype(np.float64)
+        comp_step = eq.rk4_step_f32c if f32c else eq.rk4_step_f32s
 
         def step(_, t, dt):
-            eq.rk4_step_f32c(hi, lo, t, dt)
+            comp_step(hi, lo, t, dt)
             x[:] = hi.astype(np.float64) + lo.astype(np.float64)
 
         O.integrate_const(step, obs, x, p["t0"], p["t1"], p["dt"])
@@ -89,11 +90,11 @@ def test_wkb_extrapolation_of_final_state(O, tmp_path):
 
 @pytest.mark.parametrize("stepper,eq,lam", [("classic", "frw", 0.0), ("fused", "frw", 0.0),
                                             ("fused", "lambda", 0.1), ("fused", "kg", 0.0),
-                                            ("f32c", "frw", 0.0), ("classic", "sqrt", 0.7),
-                                            ("fused", "sqrt", 0.7)])
+                                            ("f32c", "frw", 0.0), ("f32s", "frw", 0.0), ("f32s", "lambda", 0.1),
+                                            ("classic", "sqrt", 0.7), ("fused", "sqrt", 0.7)])
 def test_kg_run_matches_oracle(O, tmp_path, stepper, eq, lam):
     out, p = run_kg(tmp_path, stepper, eq, lam=lam)
-    x, obs = oracle_run(O, out, p, f32c=(stepper == "f32c"))
+    x, obs = oracle_run(O, out, p, f32c=(stepper == "f32c"), f32s=(stepper == "f32s"))
     g = np.fromfile(out + "state.dat")
     assert np.array_equal(g, x), f"final state differs ({np.count_nonzero(g != x)} entries)"
     np.testing.assert_array_equal(np.fromfile(out + "t_list.dat"), np.array(obs.t_list))
