@@ -478,11 +478,13 @@ def pacing_workload(args):
 
 
 def slab_workload(args):
-    """One GPU's share of a slab decomposition along a: the blocked step (FP32-comp, and
-    FP64 beside it) on 64 of the 512 planes of the headline lattice (KG-FRW N=512, eight
-    GPUs' worth of slabs), with four ghost planes on each side taken from the whole
-    lattice once (their refresh between steps is the exchange, not timed here). One
-    launch per step, ping-ponging two slab buffers; HIP events on the launch stream."""
+    """One GPU's share of a slab decomposition along a: the blocked step (compensated
+    storage in both forms, and FP64 beside it) on 64 of the 512 planes of the headline
+    lattice (KG-FRW N=512, eight GPUs' worth of slabs), with four ghost planes on each
+    side taken from the whole lattice once (their refresh between steps is the exchange,
+    not timed here). One launch per step (one chunk of 64 planes per tile), ping-ponging
+    two slab buffers; HIP events on the launch stream. The float-force form is paced at
+    the pace the tuner picks for the whole lattice with that chunk."""
     import torch
     import cfl_amd as cfl
     N, G = args.n, 8
@@ -495,20 +497,24 @@ def slab_workload(args):
     idx = torch.tensor([(i - 4) % N for i in range(n + 8)], device="cuda")
     slab = torch.cat([x0[: N ** 3].view(N, N * N)[idx].reshape(-1), x0[N ** 3:].view(N, N * N)[idx].reshape(-1)])
     settle(cfl, p, x0)
+    hi_w, lo_w = cfl.split_f64(x0)
     del x0
+    tw = cfl.alloc_tb_work(N, cfl.CF_PREC_F32S)
+    pace = cfl.rk4_tb_tune_pace(p, hi_w, lo_w, t0, dt, tw, n)  # kept for (N, chunk n)
+    del hi_w, lo_w, tw
     stream = torch.cuda.current_stream()
     sites = N * N * n
     paths = {}
-    for mode in ("tb_f32c", "tb_f64"):
+    for mode in ("tb_f32s", "tb_f32c", "tb_f64"):
         if mode == "tb_f64":
             bufs = [(slab,), (slab.clone(),)]
-            step = lambda k, i, o: cfl.rk4_step_tb_f64_slab(p, n, i[0], o[0], t0 + k * dt, dt)  # noqa: E731
+            step = lambda k, i, o: cfl.rk4_step_tb_f64_slab(p, n, i[0], o[0], t0 + k * dt, dt, n)  # noqa: E731
         else:
             bufs = [cfl.split_f64(slab), tuple(torch.empty(slab.numel(), dtype=torch.float32, device="cuda")
                                                for _ in range(2))]
-            step = lambda k, i, o: cfl.rk4_step_tb_f32c_slab(p, n, i[0], i[1], o[0], o[1],  # noqa: E731
-                                                             t0 + k * dt, dt)
-        if mode == "tb_f32c":  # the ghost planes of the second buffer (never written by a step)
+            f = cfl.rk4_step_tb_f32c_slab if mode == "tb_f32c" else cfl.rk4_step_tb_f32s_slab
+            step = (lambda f: lambda k, i, o: f(p, n, i[0], i[1], o[0], o[1], t0 + k * dt, dt, n))(f)  # noqa: E731
+        if mode != "tb_f64":  # the ghost planes of the second buffer (never written by a step)
             for dst, src in zip(bufs[1], bufs[0]):
                 dst.copy_(src)
         for k in range(args.warmup):
@@ -523,16 +529,18 @@ def slab_workload(args):
         ms = ev0.elapsed_time(ev1) / args.steps
         paths[mode] = {"value": sites / (ms * 1e-3) / 1e9, "ms_per_step": ms}
         bufs = step = None
-    head = paths["tb_f32c"]
+    head = paths["tb_f32s"]
     line = {
         "metric": "GSite*RK4-step/s of one GPU's slab (temporally blocked; FP32-comp, FP64 beside it)",
         "value": head["value"], "unit": "GSite*RK4-step/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "dtype": "f64",
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+        "dtype": "f32 stencil, f64 increment", "pace_ticks_per_plane": pace,
         "storage": "f32 hi + f32 compensation", "data": "synthetic (Gaussian random field, seed 2)",
         "config": {"workload": f"KG-FRW N={N}^3 L={L:g} m=1 dt=0.05, slab of {n} planes + 2 x 4 ghost planes "
                                f"(one of {G} slabs along a)", "slab_planes": n},
         "paths": paths,
-        "ghost_bytes_per_neighbour_per_step": {"tb_f32c": 4 * N * N * 16, "tb_f64": 4 * N * N * 16},
+        "ghost_bytes_per_neighbour_per_step": {"tb_f32s": 4 * N * N * 16, "tb_f32c": 4 * N * N * 16,
+                                               "tb_f64": 4 * N * N * 16},
     }
     print(json.dumps(line), flush=True)
 

@@ -196,6 +196,13 @@ def rk4_step_tb_f32c_slab(p, n_planes, hi_in, lo_in, hi_out, lo_out, t, dt, chun
                                        t, dt, chunk, _stream(stream)))
 
 
+def rk4_step_tb_f32s_slab(p, n_planes, hi_in, lo_in, hi_out, lo_out, t, dt, chunk=0, stream=None):
+    """The float-force form of rk4_step_tb_f32c_slab (paced with the pace kept for N and
+    the slab's chunk)."""
+    check(lib.cf_rk4_step_tb_f32s_slab(C.byref(p), n_planes, _ptr(hi_in), _ptr(lo_in), _ptr(hi_out), _ptr(lo_out),
+                                       t, dt, chunk, _stream(stream)))
+
+
 def rk4_step_tb_f32c(p, hi_in, lo_in, hi_out, lo_out, t, dt, chunk=0, stream=None):
     check(lib.cf_rk4_step_tb_f32c(C.byref(p), _ptr(hi_in), _ptr(lo_in), _ptr(hi_out), _ptr(lo_out), t, dt, chunk,
                                   _stream(stream)))

@@ -97,6 +97,7 @@ _SIGS = [
     ("cf_rk4_step_tb_f32c", st, [C.POINTER(KGParams), vp, vp, vp, vp, dbl, dbl, C.c_int, vp]),
     ("cf_rk4_step_tb_f64_slab", st, [C.POINTER(KGParams), i64, vp, vp, dbl, dbl, C.c_int, vp]),
     ("cf_rk4_step_tb_f32c_slab", st, [C.POINTER(KGParams), i64, vp, vp, vp, vp, dbl, dbl, C.c_int, vp]),
+    ("cf_rk4_step_tb_f32s_slab", st, [C.POINTER(KGParams), i64, vp, vp, vp, vp, dbl, dbl, C.c_int, vp]),
     ("cf_rk4_run_tb_f32c", st, [C.POINTER(KGParams), vp, vp, dbl, i64, i64, dbl, C.c_int, vp, vp]),
     ("cf_rk4_step_tb_f32s", st, [C.POINTER(KGParams), vp, vp, vp, vp, dbl, dbl, C.c_int, vp]),
     ("cf_rk4_run_tb_f32s", st, [C.POINTER(KGParams), vp, vp, dbl, i64, i64, dbl, C.c_int, vp, vp]),

@@ -371,6 +371,23 @@ static cf_status scratch_on_current_device(double **out) {
   return CF_OK;
 }
 
+static cf_status tb_step_comp_slab(const char *name, bool sa, const cf_kg_params *p, int64_t n_planes,
+                                   const float *hi_in, const float *lo_in, float *hi_out, float *lo_out, double t,
+                                   double dt, int chunk, hipStream_t st) {
+  CF_TRY(tb_check(p));
+  CF_REQUIRE(n_planes >= 1 && n_planes <= p->N, "%s: n_planes=%lld outside 1..N", name, (long long)n_planes);
+  const int64_t NA = n_planes + 2 * TBH;
+  const size_t b = 2 * (size_t)NA * p->N * p->N * sizeof(float);
+  CF_REQUIRE(hi_in && lo_in && hi_out && lo_out, "%s: null buffer", name);
+  CF_REQUIRE(disjoint(hi_in, lo_in, b) && disjoint(hi_out, lo_out, b) && disjoint(hi_in, hi_out, b) &&
+                 disjoint(hi_in, lo_out, b) && disjoint(lo_in, hi_out, b) && disjoint(lo_in, lo_out, b),
+             "%s: overlapping buffers", name);
+  TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N, TB_F32C_RB, TB_F32C_RC, n_planes), TB_F32C_RB,
+                     TB_F32C_RC, TBH, TBH + n_planes, NA);
+  if (sa) CF_TRY(tb_pace_lookup(g));  // the pace tuned for this lattice size and chunk, if any
+  return tb_launch_comp(sa, g, tb_step_host(p, t, dt), hi_in, lo_in, hi_out, lo_out, st);
+}
+
 }  // namespace cf
 
 using namespace cf;
@@ -461,21 +478,6 @@ cf_status cf_rk4_step_tb_f64_slab(const cf_kg_params *p, int64_t n_planes, const
   return tb_launch<double, false>(g, tb_step_host(p, t, dt), x_in, nullptr, x_out, nullptr, S(stream));
 }
 
-cf_status cf_rk4_step_tb_f32c_slab(const cf_kg_params *p, int64_t n_planes, const float *hi_in, const float *lo_in,
-                                   float *hi_out, float *lo_out, double t, double dt, int chunk, cf_stream stream) {
-  CF_TRY(tb_check(p));
-  CF_REQUIRE(n_planes >= 1 && n_planes <= p->N, "cf_rk4_step_tb_f32c_slab: n_planes=%lld outside 1..N",
-             (long long)n_planes);
-  const int64_t NA = n_planes + 2 * TBH;
-  const size_t b = 2 * (size_t)NA * p->N * p->N * sizeof(float);
-  CF_REQUIRE(hi_in && lo_in && hi_out && lo_out, "cf_rk4_step_tb_f32c_slab: null buffer");
-  CF_REQUIRE(disjoint(hi_in, lo_in, b) && disjoint(hi_out, lo_out, b) && disjoint(hi_in, hi_out, b) &&
-                 disjoint(hi_in, lo_out, b) && disjoint(lo_in, hi_out, b) && disjoint(lo_in, lo_out, b),
-             "cf_rk4_step_tb_f32c_slab: overlapping buffers");
-  const TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N, TB_F32C_RB, TB_F32C_RC, n_planes),
-                           TB_F32C_RB, TB_F32C_RC, TBH, TBH + n_planes, NA);
-  return tb_launch<float, true>(g, tb_step_host(p, t, dt), hi_in, lo_in, hi_out, lo_out, S(stream));
-}
 
 cf_status cf_rk4_step_tb_f32c(const cf_kg_params *p, const float *hi_in, const float *lo_in, float *hi_out,
                               float *lo_out, double t, double dt, int chunk, cf_stream stream) {
@@ -485,6 +487,18 @@ cf_status cf_rk4_step_tb_f32c(const cf_kg_params *p, const float *hi_in, const f
 cf_status cf_rk4_step_tb_f32s(const cf_kg_params *p, const float *hi_in, const float *lo_in, float *hi_out,
                               float *lo_out, double t, double dt, int chunk, cf_stream stream) {
   return tb_step_comp("cf_rk4_step_tb_f32s", true, p, hi_in, lo_in, hi_out, lo_out, t, dt, chunk, S(stream));
+}
+
+cf_status cf_rk4_step_tb_f32c_slab(const cf_kg_params *p, int64_t n_planes, const float *hi_in, const float *lo_in,
+                                   float *hi_out, float *lo_out, double t, double dt, int chunk, cf_stream stream) {
+  return tb_step_comp_slab("cf_rk4_step_tb_f32c_slab", false, p, n_planes, hi_in, lo_in, hi_out, lo_out, t, dt, chunk,
+                           S(stream));
+}
+
+cf_status cf_rk4_step_tb_f32s_slab(const cf_kg_params *p, int64_t n_planes, const float *hi_in, const float *lo_in,
+                                   float *hi_out, float *lo_out, double t, double dt, int chunk, cf_stream stream) {
+  return tb_step_comp_slab("cf_rk4_step_tb_f32s_slab", true, p, n_planes, hi_in, lo_in, hi_out, lo_out, t, dt, chunk,
+                           S(stream));
 }
 
 // Pacing of the single-precision-stencil step: time the step of this lattice at a

@@ -216,6 +216,12 @@ cf_status cf_rk4_step_tb_f64_slab(const cf_kg_params *p, int64_t n_planes, const
 cf_status cf_rk4_step_tb_f32c_slab(const cf_kg_params *p, int64_t n_planes, const float *hi_in_dev,
                                    const float *lo_in_dev, float *hi_out_dev, float *lo_out_dev, double t,
                                    double dt, int chunk, cf_stream stream);
+/* The same for the float-force form (bits of cf_rk4_step_tb_f32s). It is paced with the
+ * pace kept for (N, its chunk): tune it on the whole lattice with the same chunk
+ * (cf_rk4_tb_tune_pace) or set it (cf_rk4_tb_set_pace); otherwise it runs unpaced. */
+cf_status cf_rk4_step_tb_f32s_slab(const cf_kg_params *p, int64_t n_planes, const float *hi_in_dev,
+                                   const float *lo_in_dev, float *hi_out_dev, float *lo_out_dev, double t,
+                                   double dt, int chunk, cf_stream stream);
 
 /* Split a double state into hi/lo floats and back (x = hi + lo). */
 cf_status cf_split_f64(const double *x_dev, float *hi_dev, float *lo_dev, int64_t n, cf_stream stream);

@@ -82,26 +82,35 @@ def test_slab_one_gpu_share_n512(cfl):
     torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("mode", ["f32c", "f32s"])
 @pytest.mark.parametrize("N,sizes,lam", [(64, [16, 16, 16, 16], 0.0), (40, [13, 1, 26], 0.1)])
-def test_slab_f32c(cfl, N, sizes, lam):
-    """FP32-comp storage: hi and lo slabs, own sites equal the whole-lattice FP32-comp step."""
-    print(f"case slab f32c N={N} sizes={sizes} lam={lam}", flush=True)
+def test_slab_f32c(cfl, N, sizes, lam, mode):
+    """Compensated storage (FP64 arithmetic, or float forces paced): hi and lo slabs, own
+    sites equal the whole-lattice step of the same form."""
+    print(f"case slab {mode} N={N} sizes={sizes} lam={lam}", flush=True)
     p = cfl.kg_params(N, float(N), 1.0, lam, cfl.radiation(**FRW))
     hi, lo = cfl.split_f64(state(cfl, N, 7))
     whi, wlo = torch.empty_like(hi), torch.empty_like(lo)
-    cfl.rk4_step_tb_f32c(p, hi, lo, whi, wlo, 1.3, 0.05)
+    whole = cfl.rk4_step_tb_f32c if mode == "f32c" else cfl.rk4_step_tb_f32s
+    slab_step = cfl.rk4_step_tb_f32c_slab if mode == "f32c" else cfl.rk4_step_tb_f32s_slab
+    whole(p, hi, lo, whi, wlo, 1.3, 0.05)
+    chunk = 5
+    if mode == "f32s":
+        cfl.rk4_tb_set_pace(N, 170, chunk)  # paced slabs (timing only)
     a0 = 0
     for n in sizes:
         shi, slo = slab_of(hi, N, a0, n), slab_of(lo, N, a0, n)
         ohi = torch.full_like(shi, float("nan"))
         olo = torch.full_like(slo, float("nan"))
-        cfl.rk4_step_tb_f32c_slab(p, n, shi, slo, ohi, olo, 1.3, 0.05)
+        slab_step(p, n, shi, slo, ohi, olo, 1.3, 0.05, chunk)
         for got, whole, name in ((ohi, whi, "hi"), (olo, wlo, "lo")):
             g = got.view(2, n + 8, N * N)
             w = whole.view(2, N, N * N)
             assert torch.equal(g[:, 4:4 + n], w[:, a0:a0 + n]), f"{name} a0={a0} n={n}"
             assert torch.isnan(g[:, :4]).all() and torch.isnan(g[:, 4 + n:]).all(), "ghost planes written"
         a0 += n
+    if mode == "f32s":
+        cfl.rk4_tb_set_pace(N, 0, chunk)
 
 
 def test_slab_rejects_bad_sizes(cfl):
