@@ -55,6 +55,12 @@ __device__ __forceinline__ TbUnit tb_unit_of(const TbGeom &g, int bid) {
 // at N=512 (GSite*step/s, same bits): 32 x 32 (24 x 24 tiles) 87.8-88.3; 24 x 40 (16 x 32
 // tiles, whose c edges fall on 128-byte lines; 960 threads) 83.0-83.4; 25 x 40 81.4-82.1.
 constexpr int TB_F32C_RB = 32, TB_F32C_RC = 32;
+// Region shape of the single-precision-stencil kernel: 24 x 40 (16 x 32 tiles, 960 of the
+// 1024 threads; the tiles' c edges fall on 128-byte lines, and N=512 makes 512 tiles,
+// 1024 blocks in two chunks, four full rounds over 256 CUs). Measured at N=512
+// (GSite*step/s, same bits): unpaced 101.7 against 76.7 for 32 x 32, paced 107.0 (at 115
+// ticks per plane) against 97.3 (at 128).
+constexpr int TB_F32S_RB = 24, TB_F32S_RC = 40;
 
 template <class T, bool F32C, bool LAM>
 __global__ __launch_bounds__(1024) void k_rk4_tb(TbGeom g, TbStep k, const T *__restrict__ xin,
@@ -71,7 +77,7 @@ __global__ __launch_bounds__(1024) void k_rk4_tb(TbGeom g, TbStep k, const T *__
 }
 
 // The compensated storage with the single-precision stencil (float forces, double
-// increment): the same body and geometry as the FP32-comp kernel; PACE: paced
+// increment): the same body as the FP32-comp kernel over 24 x 40 regions; PACE: paced
 // (cf_tb_body.hpp), at g.pace ticks per plane iteration. Occupancy stays at one
 // 1024-thread block per CU, so the compiler may use the 128 VGPRs that allows: with
 // its default target it kept 84 and serialised the LDS neighbour reads (read, wait,
@@ -82,7 +88,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
     float *__restrict__ xout, float *__restrict__ lout) {
   __shared__ TbLds<float> lds;
   const TbUnit u = tb_unit_of(g, tb_xcd_order(blockIdx.x, g.total));
-  tb_unit<float, true, LAM, true, false, TB_KG, false, TB_F32C_RB, TB_F32C_RC, true, PACE>(lds, u, k, xin, lin, xout,
+  tb_unit<float, true, LAM, true, false, TB_KG, false, TB_F32S_RB, TB_F32S_RC, true, PACE>(lds, u, k, xin, lin, xout,
                                                                                        lout, nullptr, TbPsi{}, kf);
 }
 
@@ -179,6 +185,11 @@ static float *same_residue(void *from, const float *like) {
 // Geometry of the FP32-comp kernel (its region shape), chunk <= 0: the cost model's.
 static TbGeom tb_f32c_geom(int64_t N, int chunk) {
   return tb_geom(N, chunk > 0 ? chunk : tb_default_chunk(N, TB_F32C_RB, TB_F32C_RC), TB_F32C_RB, TB_F32C_RC);
+}
+// ... and of the single-precision-stencil kernel (sa) or the FP32-comp one.
+static TbGeom tb_comp_geom(bool sa, int64_t N, int chunk) {
+  if (!sa) return tb_f32c_geom(N, chunk);
+  return tb_geom(N, chunk > 0 ? chunk : tb_default_chunk(N, TB_F32S_RB, TB_F32S_RC), TB_F32S_RB, TB_F32S_RC);
 }
 
 template <class T, bool F32C>
@@ -315,7 +326,7 @@ static cf_status tb_run_comp(const char *name, bool sa, const cf_kg_params *p, f
                  disjoint(hi, n * sizeof(float), work, wb) && disjoint(lo, n * sizeof(float), work, wb),
              "%s: null or overlapping buffer", name);
   CF_REQUIRE(n_steps >= 0, "%s: negative step count", name);
-  TbGeom g = tb_f32c_geom(p->N, chunk);
+  TbGeom g = tb_comp_geom(sa, p->N, chunk);
   if (sa) CF_TRY(tb_pace_lookup(g));
   float *his[2] = {hi, same_residue(work, hi)};
   float *los[2] = {lo, same_residue(his[1] + n, lo)};
@@ -340,7 +351,7 @@ static cf_status tb_step_comp(const char *name, bool sa, const cf_kg_params *p, 
   CF_REQUIRE(disjoint(hi_in, lo_in, b) && disjoint(hi_out, lo_out, b) && disjoint(hi_in, hi_out, b) &&
                  disjoint(hi_in, lo_out, b) && disjoint(lo_in, hi_out, b) && disjoint(lo_in, lo_out, b),
              "%s: overlapping buffers", name);
-  TbGeom g = tb_f32c_geom(p->N, chunk);
+  TbGeom g = tb_comp_geom(sa, p->N, chunk);
   if (sa) CF_TRY(tb_pace_lookup(g));
   return tb_launch_comp(sa, g, tb_step_host(p, t, dt), hi_in, lo_in, hi_out, lo_out, st);
 }
@@ -382,8 +393,9 @@ static cf_status tb_step_comp_slab(const char *name, bool sa, const cf_kg_params
   CF_REQUIRE(disjoint(hi_in, lo_in, b) && disjoint(hi_out, lo_out, b) && disjoint(hi_in, hi_out, b) &&
                  disjoint(hi_in, lo_out, b) && disjoint(lo_in, hi_out, b) && disjoint(lo_in, lo_out, b),
              "%s: overlapping buffers", name);
-  TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N, TB_F32C_RB, TB_F32C_RC, n_planes), TB_F32C_RB,
-                     TB_F32C_RC, TBH, TBH + n_planes, NA);
+  const int RB = sa ? TB_F32S_RB : TB_F32C_RB, RC = sa ? TB_F32S_RC : TB_F32C_RC;
+  TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N, RB, RC, n_planes), RB, RC, TBH,
+                     TBH + n_planes, NA);
   if (sa) CF_TRY(tb_pace_lookup(g));  // the pace tuned for this lattice size and chunk, if any
   return tb_launch_comp(sa, g, tb_step_host(p, t, dt), hi_in, lo_in, hi_out, lo_out, st);
 }
@@ -519,7 +531,7 @@ cf_status cf_rk4_tb_tune_pace(const cf_kg_params *p, const float *hi, const floa
                  disjoint(hi, n * sizeof(float), work, wb) && disjoint(lo, n * sizeof(float), work, wb),
              "cf_rk4_tb_tune_pace: null or overlapping buffer");
   hipStream_t st = S(stream);
-  TbGeom g = tb_f32c_geom(p->N, chunk);
+  TbGeom g = tb_comp_geom(true, p->N, chunk);
   float *ho = same_residue(work, hi), *lw = same_residue(ho + n, lo);
   const TbStep k = tb_step_host(p, t, dt);
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -578,12 +590,12 @@ cf_status cf_rk4_tb_tune_pace(const cf_kg_params *p, const float *hi, const floa
 cf_status cf_rk4_tb_set_pace(int64_t N, int chunk, int pace) {
   CF_REQUIRE(valid_N(N), "cf_rk4_tb_set_pace: unsupported lattice size N=%lld", (long long)N);
   CF_REQUIRE(pace >= 0 && pace <= 100000, "cf_rk4_tb_set_pace: pace %d outside 0..100000", pace);
-  return tb_pace_store(N, tb_f32c_geom(N, chunk).A, pace);
+  return tb_pace_store(N, tb_comp_geom(true, N, chunk).A, pace);
 }
 
 int cf_rk4_tb_get_pace(int64_t N, int chunk) {
   if (!valid_N(N)) return -1;
-  TbGeom g = tb_f32c_geom(N, chunk);
+  TbGeom g = tb_comp_geom(true, N, chunk);
   if (tb_pace_lookup(g) != CF_OK) return -1;
   return g.pace;
 }
