@@ -493,7 +493,7 @@ static cf_status ens_time(cf_ensemble *e, int reps, double dt, float *ms) {
 
 // CF_PREC_F32S: the pace of the units' plane iterations, as the single-lattice tuner
 // (cf_rk4_tb_tune_pace) picks it: a coarse grid of paces, then tick by tick around the
-// best, each pace P scored by the slowest of P-4..P (a margin above the cliff below the
+// best, each pace P scored by the slowest of P-2..P (a margin above the fall below the
 // fastest pace); unpaced if that is as fast.
 static cf_status ens_tune_pace(cf_ensemble *e, int reps, double dt) {
   std::map<int, float> ms;
@@ -509,7 +509,7 @@ static cf_status ens_tune_pace(cf_ensemble *e, int reps, double dt) {
   int coarse = 0;
   for (const auto &kv : ms)
     if (kv.second < ms[coarse]) coarse = kv.first;
-  constexpr int MARGIN = 4;
+  constexpr int MARGIN = 2;
   if (coarse > 0)
     for (int pace = coarse - 8; pace <= coarse + 8 + MARGIN; ++pace) CF_TRY(time_pace(pace));
   int pick = 0;

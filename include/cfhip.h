@@ -165,10 +165,10 @@ cf_status cf_rk4_run_tb_f32s(const cf_kg_params *p, float *hi_dev, float *lo_dev
  * then read their shared halo rows while these are still in the L2). The pace, in ticks
  * of the GPU's 100 MHz real-time counter per plane iteration (0: unpaced), is kept per
  * device, lattice size and chunk and used by cf_rk4_run_tb_f32s and cf_rk4_step_tb_f32s.
- * cf_rk4_tb_tune_pace times about 200 steps of the lattice of p at paces 0..3.2 us per
+ * cf_rk4_tb_tune_pace times about 350 steps of the lattice of p at paces 0..3.2 us per
  * plane (state hi/lo at time t, unchanged; work as for cf_rk4_run_tb_f32s) and keeps
- * the pace P whose slowest of P-4..P is fastest (the rate falls off a cliff just below
- * the best pace, so the pick keeps a margin above it; unpaced if that is as fast), and
+ * the pace P whose slowest of P-2..P is fastest (the rate falls just below the best
+ * pace, so the pick keeps a margin above it; unpaced if that is as fast), and
  * returns it in *pace_out (may be null); it synchronises the stream. Pacing changes
  * timing only, never results. cf_rk4_tb_get_pace returns -1 for an unsupported N. */
 cf_status cf_rk4_tb_tune_pace(const cf_kg_params *p, const float *hi_dev, const float *lo_dev, double t, double dt,
