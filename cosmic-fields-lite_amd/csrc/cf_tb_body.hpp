@@ -86,6 +86,7 @@ struct TbPsi {
 // pace * j ticks of the 100 MHz real-time counter have passed since the block began,
 // so blocks cannot run ahead of the schedule (blocks that fall behind it catch up at
 // full speed). Pacing changes timing only, never results.
+
 // One work unit: lattice size, output tile origin and edges (T along b, Tc along
 // c), plane chunk, and the planes NA that the state buffers hold along a (N for a
 // periodic lattice; a slab of a larger lattice holds its planes plus ghost planes,
@@ -95,7 +96,7 @@ struct TbUnit {
   int64_t b0, c0, a0, a1;
   int T, Tc;
   int64_t NA;
-  int pace;  // ticks per iteration (bodies built with PACE)
+  int pace;  // ticks per plane iteration (bodies built with PACE; see Pacing above)
 };
 
 __device__ __forceinline__ double tb_rhs(double phi, double pa1, double pam, double pb1, double pbm, double pc1,
