@@ -30,6 +30,7 @@ struct TbGeom {
   int64_t NA;        // planes of the state buffers along a (N, or a slab's planes + ghosts)
   int64_t a_lo, a_hi;  // output planes [a_lo, a_hi)
   int pace;            // ticks of the 100 MHz real-time counter per plane iteration, 0: unpaced
+  int rb, rc;          // region shape (Tb, Tc plus the halo) the launch was planned for
 };
 
 // Work unit of logical block `bid`: tile (tb, tc), chunk ac, c fastest.
@@ -82,14 +83,14 @@ __global__ __launch_bounds__(1024) void k_rk4_tb(TbGeom g, TbStep k, const T *__
 // 1024-thread block per CU, so the compiler may use the 128 VGPRs that allows: with
 // its default target it kept 84 and serialised the LDS neighbour reads (read, wait,
 // add, read), 77.6 against 80.4 GSite*step/s unpaced at N=512.
-template <bool LAM, bool PACE>
+template <bool LAM, bool PACE, int RB = TB_F32S_RB, int RC = TB_F32S_RC>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_rk4_tb_sa(
     TbGeom g, TbStep k, TbStepF kf, const float *__restrict__ xin, const float *__restrict__ lin,
     float *__restrict__ xout, float *__restrict__ lout) {
   __shared__ TbLds<float> lds;
   const TbUnit u = tb_unit_of(g, tb_xcd_order(blockIdx.x, g.total));
-  tb_unit<float, true, LAM, true, false, TB_KG, false, TB_F32S_RB, TB_F32S_RC, true, PACE>(lds, u, k, xin, lin, xout,
-                                                                                       lout, nullptr, TbPsi{}, kf);
+  tb_unit<float, true, LAM, true, false, TB_KG, false, RB, RC, true, PACE>(lds, u, k, xin, lin, xout, lout, nullptr,
+                                                                         TbPsi{}, kf);
 }
 
 // FP64: the same step with the state streamed into an LDS ring TB_RING_DEPTH planes
@@ -133,6 +134,8 @@ static TbGeom tb_geom(int64_t N, int A, int RB = TBR, int RC = TBR, int64_t a_lo
   g.n_ac = (int)cdiv(planes, g.A);
   g.total = g.ntb * g.ntc * g.n_ac;
   g.pace = 0;
+  g.rb = RB;
+  g.rc = RC;
   return g;
 }
 
@@ -145,7 +148,8 @@ static TbGeom tb_geom(int64_t N, int A, int RB = TBR, int RC = TBR, int64_t a_lo
 // 128 103 / 89 against 65 / 51; N=384 one chunk 118 / 88 against 92 / 74 with 256;
 // N=512 chunk 256 (the rule's pick) and 171 within 1-3% of each other, one chunk
 // 96 / 77.
-static int tb_default_chunk(int64_t N, int RB = TBR, int RC = TBR, int64_t planes = -1) {
+static int tb_default_chunk(int64_t N, int RB = TBR, int RC = TBR, int64_t planes = -1,
+                            int64_t *cost_out = nullptr) {
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
@@ -166,6 +170,7 @@ static int tb_default_chunk(int64_t N, int RB = TBR, int RC = TBR, int64_t plane
   int64_t pick = 1;
   for (int64_t k = 1; k <= kmax; ++k)
     if (cost[k] >= 0 && cost[k] * 100 <= best * 102) pick = k;
+  if (cost_out) *cost_out = best;
   return (int)cdiv(planes, pick);
 }
 
@@ -186,10 +191,27 @@ static float *same_residue(void *from, const float *like) {
 static TbGeom tb_f32c_geom(int64_t N, int chunk) {
   return tb_geom(N, chunk > 0 ? chunk : tb_default_chunk(N, TB_F32C_RB, TB_F32C_RC), TB_F32C_RB, TB_F32C_RC);
 }
+// Region shape of the single-precision-stencil kernel for `planes` output planes of an
+// N^2 cross-section: 24 x 40 unless the cost model's launch over 32 x 32 regions is more
+// than TB_F32S_SQUARE_GAIN percent cheaper (at N=192 and 384 its 8 x 8 / 16 x 16 tiles
+// fill the CUs in one round where 24 x 40 needs more rounds of shorter chunks; measured
+// at N=384: 105.7 GSite*step/s against 97.3). Elsewhere 24 x 40 is faster at equal cost
+// (N=256: 109.8 against 95.4; 768: 108.3 against 92.6, a cost ratio of 1.14).
+constexpr int TB_F32S_SQUARE_GAIN = 20;
+static void tb_f32s_shape(int64_t N, int64_t planes, int *RB, int *RC) {
+  int64_t c_strip = 0, c_square = 0;
+  tb_default_chunk(N, TB_F32S_RB, TB_F32S_RC, planes, &c_strip);
+  tb_default_chunk(N, TB_F32C_RB, TB_F32C_RC, planes, &c_square);
+  const bool square = c_square * (100 + TB_F32S_SQUARE_GAIN) < c_strip * 100;
+  *RB = square ? TB_F32C_RB : TB_F32S_RB;
+  *RC = square ? TB_F32C_RC : TB_F32S_RC;
+}
 // ... and of the single-precision-stencil kernel (sa) or the FP32-comp one.
 static TbGeom tb_comp_geom(bool sa, int64_t N, int chunk) {
   if (!sa) return tb_f32c_geom(N, chunk);
-  return tb_geom(N, chunk > 0 ? chunk : tb_default_chunk(N, TB_F32S_RB, TB_F32S_RC), TB_F32S_RB, TB_F32S_RC);
+  int RB = 0, RC = 0;
+  tb_f32s_shape(N, N, &RB, &RC);
+  return tb_geom(N, chunk > 0 ? chunk : tb_default_chunk(N, RB, RC), RB, RC);
 }
 
 template <class T, bool F32C>
@@ -225,18 +247,30 @@ static TbStepF tb_step_f32(const TbStep &k) {
   return f;
 }
 
+template <int RB, int RC>
+static void tb_launch_sa_shape(const TbGeom &g, const TbStep &k, const TbStepF &kf, const float *xin,
+                               const float *lin, float *xout, float *lout, hipStream_t st) {
+  const bool lam = k.lam != 0.0, paced = g.pace > 0;
+  const dim3 grid(g.total), block(1024);
+  if (lam && paced)
+    hipLaunchKernelGGL((k_rk4_tb_sa<true, true, RB, RC>), grid, block, 0, st, g, k, kf, xin, lin, xout, lout);
+  else if (lam)
+    hipLaunchKernelGGL((k_rk4_tb_sa<true, false, RB, RC>), grid, block, 0, st, g, k, kf, xin, lin, xout, lout);
+  else if (paced)
+    hipLaunchKernelGGL((k_rk4_tb_sa<false, true, RB, RC>), grid, block, 0, st, g, k, kf, xin, lin, xout, lout);
+  else
+    hipLaunchKernelGGL((k_rk4_tb_sa<false, false, RB, RC>), grid, block, 0, st, g, k, kf, xin, lin, xout, lout);
+}
+
 static cf_status tb_launch_sa(const TbGeom &g, const TbStep &k, const float *xin, const float *lin, float *xout,
                               float *lout, hipStream_t st) {
   const TbStepF kf = tb_step_f32(k);
-  const bool lam = k.lam != 0.0, paced = g.pace > 0;
-  if (lam && paced)
-    hipLaunchKernelGGL((k_rk4_tb_sa<true, true>), dim3(g.total), dim3(1024), 0, st, g, k, kf, xin, lin, xout, lout);
-  else if (lam)
-    hipLaunchKernelGGL((k_rk4_tb_sa<true, false>), dim3(g.total), dim3(1024), 0, st, g, k, kf, xin, lin, xout, lout);
-  else if (paced)
-    hipLaunchKernelGGL((k_rk4_tb_sa<false, true>), dim3(g.total), dim3(1024), 0, st, g, k, kf, xin, lin, xout, lout);
+  if (g.rb == TB_F32S_RB && g.rc == TB_F32S_RC)
+    tb_launch_sa_shape<TB_F32S_RB, TB_F32S_RC>(g, k, kf, xin, lin, xout, lout, st);
+  else if (g.rb == TB_F32C_RB && g.rc == TB_F32C_RC)
+    tb_launch_sa_shape<TB_F32C_RB, TB_F32C_RC>(g, k, kf, xin, lin, xout, lout, st);
   else
-    hipLaunchKernelGGL((k_rk4_tb_sa<false, false>), dim3(g.total), dim3(1024), 0, st, g, k, kf, xin, lin, xout, lout);
+    CF_REQUIRE(false, "single-precision stencil: no kernel for %d x %d regions", g.rb, g.rc);
   CF_LAUNCHED();
   return CF_OK;
 }
@@ -286,25 +320,26 @@ cf_status tb_step_variant(int var, const cf_kg_params *p, double f_a, const doub
 // of the 100 MHz real-time counter per plane iteration; absent or 0: unpaced), set by
 // cf_rk4_tb_tune_pace or cf_rk4_tb_set_pace.
 static std::mutex g_pace_mu;
-static std::map<std::tuple<int, int64_t, int>, int> g_pace;
+using TbPaceKey = std::tuple<int, int64_t, int, int>;  // device, N, chunk, region rows
+static std::map<TbPaceKey, int> g_pace;
 
-static cf_status tb_pace_key(int64_t N, int A, std::tuple<int, int64_t, int> *key) {
+static cf_status tb_pace_key(const TbGeom &g, TbPaceKey *key) {
   int dev = 0;
   CF_HIP(hipGetDevice(&dev));
-  *key = std::make_tuple(dev, N, A);
+  *key = std::make_tuple(dev, g.N, g.A, g.rb);
   return CF_OK;
 }
 static cf_status tb_pace_lookup(TbGeom &g) {
-  std::tuple<int, int64_t, int> key;
-  CF_TRY(tb_pace_key(g.N, g.A, &key));
+  TbPaceKey key;
+  CF_TRY(tb_pace_key(g, &key));
   std::lock_guard<std::mutex> lock(g_pace_mu);
   const auto it = g_pace.find(key);
   g.pace = it == g_pace.end() ? 0 : it->second;
   return CF_OK;
 }
-static cf_status tb_pace_store(int64_t N, int A, int pace) {
-  std::tuple<int, int64_t, int> key;
-  CF_TRY(tb_pace_key(N, A, &key));
+static cf_status tb_pace_store(const TbGeom &g, int pace) {
+  TbPaceKey key;
+  CF_TRY(tb_pace_key(g, &key));
   std::lock_guard<std::mutex> lock(g_pace_mu);
   g_pace[key] = pace;
   return CF_OK;
@@ -393,7 +428,8 @@ static cf_status tb_step_comp_slab(const char *name, bool sa, const cf_kg_params
   CF_REQUIRE(disjoint(hi_in, lo_in, b) && disjoint(hi_out, lo_out, b) && disjoint(hi_in, hi_out, b) &&
                  disjoint(hi_in, lo_out, b) && disjoint(lo_in, hi_out, b) && disjoint(lo_in, lo_out, b),
              "%s: overlapping buffers", name);
-  const int RB = sa ? TB_F32S_RB : TB_F32C_RB, RC = sa ? TB_F32S_RC : TB_F32C_RC;
+  int RB = TB_F32C_RB, RC = TB_F32C_RC;
+  if (sa) tb_f32s_shape(p->N, n_planes, &RB, &RC);
   TbGeom g = tb_geom(p->N, chunk > 0 ? chunk : tb_default_chunk(p->N, RB, RC, n_planes), RB, RC, TBH,
                      TBH + n_planes, NA);
   if (sa) CF_TRY(tb_pace_lookup(g));  // the pace tuned for this lattice size and chunk, if any
@@ -587,7 +623,7 @@ cf_status cf_rk4_tb_tune_pace(const cf_kg_params *p, const float *hi, const floa
       pick_worst = worst;
     }
   }
-  CF_TRY(tb_pace_store(p->N, g.A, pick));
+  CF_TRY(tb_pace_store(g, pick));
   if (pace_out) *pace_out = pick;
   return CF_OK;
 }
@@ -595,7 +631,7 @@ cf_status cf_rk4_tb_tune_pace(const cf_kg_params *p, const float *hi, const floa
 cf_status cf_rk4_tb_set_pace(int64_t N, int chunk, int pace) {
   CF_REQUIRE(valid_N(N), "cf_rk4_tb_set_pace: unsupported lattice size N=%lld", (long long)N);
   CF_REQUIRE(pace >= 0 && pace <= 100000, "cf_rk4_tb_set_pace: pace %d outside 0..100000", pace);
-  return tb_pace_store(N, tb_comp_geom(true, N, chunk).A, pace);
+  return tb_pace_store(tb_comp_geom(true, N, chunk), pace);
 }
 
 int cf_rk4_tb_get_pace(int64_t N, int chunk) {

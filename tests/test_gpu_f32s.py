@@ -2,7 +2,9 @@
 cf_rk4_step_tb_f32s) against the oracle's orc_rk4_step_f32s: bit-identical hi and lo
 for N below the tile, partial edge tiles, explicit and default chunks, lambda on and
 off, and odd step counts (which end with the ping-pong copy). Paced and unpaced
-launches give the same bits, and the pace tuner leaves the state alone."""
+launches give the same bits, and the pace tuner leaves the state alone. On a 256-CU GPU
+the launch cost model runs N=192 over 32 x 32 regions and every other size here over
+24 x 40, so both instantiations of the kernel are checked."""
 import numpy as np
 import pytest
 import torch
@@ -20,7 +22,8 @@ def split(x):
 
 
 @pytest.mark.parametrize("N,lam,steps,chunk", [(4, 0.0, 2, 0), (6, 0.1, 1, 0), (16, 0.0, 3, 0), (40, 0.1, 3, 7),
-                                               (64, 0.1, 2, 0), (96, 0.0, 1, 0), (128, 0.0, 2, 0)])
+                                               (64, 0.1, 2, 0), (96, 0.0, 1, 0), (128, 0.0, 2, 0),
+                                               (192, 0.1, 1, 0), (192, 0.0, 2, 24)])
 def test_f32s_bitwise_oracle(cfl, O, N, lam, steps, chunk):
     print(f"case tb f32s N={N} lam={lam} steps={steps} chunk={chunk}", flush=True)
     L = float(N)
@@ -50,9 +53,9 @@ def test_f32s_steps_equal_run(cfl):
     assert torch.equal(bufs[1][0], hd) and torch.equal(bufs[1][1], ld)
 
 
-def test_f32s_paced_same_bits(cfl):
+@pytest.mark.parametrize("N", [96, 192])
+def test_f32s_paced_same_bits(cfl, N):
     """Pacing (here far slower than the kernel's own pace) changes timing only."""
-    N = 96
     print(f"case f32s paced N={N}", flush=True)
     p = cfl.kg_params(N, float(N), 1.0, 0.0, cfl.radiation(**FRW))
     hi, lo = split(rand_state(N, 25, amp=0.5))
