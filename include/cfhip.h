@@ -157,14 +157,18 @@ cf_status cf_rk4_run_tb_f32c(const cf_kg_params *p, float *hi_dev, float *lo_dev
  * (hi + lo) + (increment + dt/6 k4) is formed in double and split into hi and lo. The
  * increment weights stay double, so the scheme has no drift from rounded step
  * lengths. Work buffer and placement as cf_rk4_run_tb_f32c
- * (cf_rk4_tb_work_bytes(N, CF_PREC_F32S)). */
+ * (cf_rk4_tb_work_bytes(N, CF_PREC_F32S)). The launch covers the lattice with 24 x 40
+ * or 32 x 32 regions (16 x 32 or 24 x 24 tiles), whichever the chunk cost model favours
+ * for N and the device's CU count; both give the same bits. */
 cf_status cf_rk4_run_tb_f32s(const cf_kg_params *p, float *hi_dev, float *lo_dev, double t0, int64_t k0,
                              int64_t n_steps, double dt, int chunk, void *work_dev, cf_stream stream);
 /* Pacing of the single-precision-stencil step (no reference counterpart). Its blocks
  * run faster when held to a common schedule of plane iterations (neighbouring tiles
  * then read their shared halo rows while these are still in the L2). The pace, in ticks
  * of the GPU's 100 MHz real-time counter per plane iteration (0: unpaced), is kept per
- * device, lattice size and chunk and used by cf_rk4_run_tb_f32s and cf_rk4_step_tb_f32s.
+ * device, lattice size, chunk and region shape and used by cf_rk4_run_tb_f32s and
+ * cf_rk4_step_tb_f32s (and by cf_rk4_step_tb_f32s_slab where its slab gets the same
+ * chunk and shape).
  * cf_rk4_tb_tune_pace times about 350 steps of the lattice of p at paces 0..3.2 us per
  * plane (state hi/lo at time t, unchanged; work as for cf_rk4_run_tb_f32s) and keeps
  * the pace P whose slowest of P-2..P is fastest (the rate falls just below the best
