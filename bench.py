@@ -55,7 +55,7 @@ KERNEL_NAMES = {"f64": "cf::k_march<double, 1, S, false> (4 stage kernels)",
                 "f32c": "cf::k_march<float, 2, S, false> (4 stage kernels)",
                 "tb_f64": "cf::k_rk4_tb_ring<false>",
                 "tb_f32c": "cf::k_rk4_tb<float, true, false>",
-                "tb_f32s": "cf::k_rk4_tb_sa<false, true>"}  # paced; <false, false> when the tuner picks 0
+                "tb_f32s": "cf::k_rk4_tb_sa<false, {paced}, {rows}, {cols}>"}  # region shape per N
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "rk4_tb_N512_pmc_traffic.md")
 # What bounds each path's kernel at N=512, from the committed PMC summaries: the FP32-comp
 # blocked kernel issues VALU 76% of its cycles (float <-> double conversions on top of the
@@ -426,7 +426,7 @@ def pacing_workload(args):
     """The single-precision-stencil step (f32s) of the N=512 FRW workload at each pace of
     --paces and at the tuner's pick, --steps timed steps per pace after 4 untimed ones,
     two passes over the list. Paced and unpaced launches are different kernels
-    (k_rk4_tb_sa<LAM, true> / <LAM, false>), so a profiler separates them."""
+    (k_rk4_tb_sa<LAM, true, ...> / <LAM, false, ...>), so a profiler separates them."""
     import torch
     import cfl_amd as cfl
     N, L, dt, t0 = args.n, float(args.n), 0.05, 1.0
@@ -715,8 +715,9 @@ def main():
         canon = BYTES_PER_SITE_STEP[mode.replace("tb_", "")]
         kb = DESIGN_BYTES_PER_SITE_STEP[mode]
         kname = KERNEL_NAMES[mode]
-        if mode == "tb_f32s" and pace == 0:
-            kname = "cf::k_rk4_tb_sa<false, false>"
+        if mode == "tb_f32s":
+            rows, cols = cfl.rk4_tb_f32s_region(N)
+            kname = kname.format(paced="true" if pace > 0 else "false", rows=rows, cols=cols)
         results[mode] = {"value": rate, "ms_per_step": ms, "wall_s": wall,
                          "kernel": kname,
                          "kernel_bytes_per_site_step": kb,

@@ -177,6 +177,13 @@ def rk4_tb_get_pace(N, chunk=0):
     return lib.cf_rk4_tb_get_pace(N, chunk)
 
 
+def rk4_tb_f32s_region(N):
+    """(rows, columns) of the regions the f32s step covers an N^3 lattice with."""
+    rows, cols = C.c_int(0), C.c_int(0)
+    check(lib.cf_rk4_tb_f32s_region(N, C.byref(rows), C.byref(cols)))
+    return rows.value, cols.value
+
+
 def rk4_step_tb_f64(p, x_in, x_out, t, dt, chunk=0, stream=None):
     """One temporally blocked step out of place: x_out = step(x_in)."""
     check(lib.cf_rk4_step_tb_f64(C.byref(p), _ptr(x_in), _ptr(x_out), t, dt, chunk, _stream(stream)))

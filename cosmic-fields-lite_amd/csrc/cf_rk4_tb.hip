@@ -634,6 +634,13 @@ cf_status cf_rk4_tb_set_pace(int64_t N, int chunk, int pace) {
   return tb_pace_store(tb_comp_geom(true, N, chunk), pace);
 }
 
+cf_status cf_rk4_tb_f32s_region(int64_t N, int *rows, int *cols) {
+  CF_REQUIRE(valid_N(N), "cf_rk4_tb_f32s_region: unsupported lattice size N=%lld", (long long)N);
+  CF_REQUIRE(rows && cols, "cf_rk4_tb_f32s_region: null output");
+  tb_f32s_shape(N, N, rows, cols);
+  return CF_OK;
+}
+
 int cf_rk4_tb_get_pace(int64_t N, int chunk) {
   if (!valid_N(N)) return -1;
   TbGeom g = tb_comp_geom(true, N, chunk);

@@ -179,6 +179,10 @@ cf_status cf_rk4_tb_tune_pace(const cf_kg_params *p, const float *hi_dev, const 
                               int chunk, void *work_dev, cf_stream stream, int *pace_out);
 cf_status cf_rk4_tb_set_pace(int64_t N, int chunk, int pace);
 int cf_rk4_tb_get_pace(int64_t N, int chunk);
+/* Region shape (rows along b x columns along c, halo included) over which
+ * cf_rk4_run_tb_f32s and cf_rk4_step_tb_f32s cover an N^3 lattice on the current device:
+ * 24 x 40 or 32 x 32 (no reference counterpart; the kernel's name carries it). */
+cf_status cf_rk4_tb_f32s_region(int64_t N, int *rows, int *cols);
 /* cf_rk4_run_tb_f64 without a work buffer and without any allocation. No counterpart
  * in the reference: it is for callers that cannot allocate device memory (code
  * under hipGraph capture, serving loops, harnesses that forbid hipMalloc). The

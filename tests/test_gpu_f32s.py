@@ -38,6 +38,19 @@ def test_f32s_bitwise_oracle(cfl, O, N, lam, steps, chunk):
     assert_bitwise(host(ld), lo, "tb f32s lo")
 
 
+def test_f32s_region_choice(cfl):
+    """The region shape follows the launch cost model: 32 x 32 where its tiles fill the
+    CUs in fewer rounds (N=192, 384 on 256 CUs), 24 x 40 elsewhere."""
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    print(f"case f32s region choice cus={cus}", flush=True)
+    for N in (4, 16, 100, 512):
+        assert cfl.rk4_tb_f32s_region(N) in ((24, 40), (32, 32))
+    if cus == 256:
+        got = {N: cfl.rk4_tb_f32s_region(N) for N in (128, 192, 256, 384, 512, 768, 1024)}
+        assert got == {128: (24, 40), 192: (32, 32), 256: (24, 40), 384: (32, 32), 512: (24, 40),
+                       768: (24, 40), 1024: (24, 40)}, got
+
+
 def test_f32s_steps_equal_run(cfl):
     """Three out-of-place steps equal a three-step run."""
     N = 64
