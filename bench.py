@@ -442,6 +442,7 @@ def pacing_workload(args):
     t_tune = time.perf_counter()
     tuned = cfl.rk4_tb_tune_pace(p, hi, lo, t0, dt, work, args.chunk)
     tune_s = time.perf_counter() - t_tune
+    chunk = args.chunk if args.chunk > 0 else cfl.rk4_tb_get_chunk(N)  # the tuner's pick for 0
     del work
     paces = [int(q) for q in args.paces.split(",") if q] + [tuned]
     stream = torch.cuda.current_stream()
@@ -467,6 +468,7 @@ def pacing_workload(args):
     line = {
         "metric": "lattice site-steps per second of the f32s step by pace",
         "value": max(rates[str(tuned)]), "unit": "GSite*RK4-step/s at the tuned pace", "higher_is_better": True,
+        "chunk": chunk,
         "n_gpus": 1, "steps": args.steps, "dtype": "f32 stencil, f64 increment",
         "data": "synthetic (Gaussian random field, seed 2)",
         "config": {"workload": f"KG-FRW radiation era N={N}^3 L={L:g} m=1 dt=0.05 RK4, f32s step",
@@ -727,6 +729,7 @@ def main():
         if mode == "tb_f32s":
             results[mode]["pace_ticks_per_plane"] = pace
             results[mode]["pace_tune_s"] = tune_s
+            results[mode]["chunk"] = args.chunk if args.chunk > 0 else cfl.rk4_tb_get_chunk(N)
         print(json.dumps({"path": mode, **results[mode]}), file=sys.stderr, flush=True)
         run = step = bufs = work = None
         torch.cuda.synchronize()

@@ -162,7 +162,9 @@ def rk4_run_tb_f32s(p, hi, lo, t0, k0, n_steps, dt, work, chunk=0, stream=None):
 
 def rk4_tb_tune_pace(p, hi, lo, t, dt, work, chunk=0, stream=None):
     """Time the single-precision-stencil step of this lattice at a range of paces and
-    keep the best for its later runs and steps (state unchanged); returns the pace."""
+    keep the best for its later runs and steps (state unchanged); returns the pace. With
+    chunk 0 the near-best chunk lengths are tried too and the fastest becomes N's
+    default chunk (rk4_tb_get_chunk)."""
     out = C.c_int(0)
     check(lib.cf_rk4_tb_tune_pace(C.byref(p), _ptr(hi), _ptr(lo), t, dt, chunk, _ptr(work), _stream(stream),
                                   C.byref(out)))
@@ -175,6 +177,15 @@ def rk4_tb_set_pace(N, pace, chunk=0):
 
 def rk4_tb_get_pace(N, chunk=0):
     return lib.cf_rk4_tb_get_pace(N, chunk)
+
+
+def rk4_tb_set_chunk(N, chunk):
+    """Default chunk of the f32s step for N on this device (0: the cost model's)."""
+    check(lib.cf_rk4_tb_set_chunk(N, chunk))
+
+
+def rk4_tb_get_chunk(N):
+    return lib.cf_rk4_tb_get_chunk(N)
 
 
 def rk4_tb_f32s_region(N):

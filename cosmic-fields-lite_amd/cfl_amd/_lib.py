@@ -105,6 +105,8 @@ _SIGS = [
     ("cf_rk4_tb_set_pace", st, [i64, C.c_int, C.c_int]),
     ("cf_rk4_tb_get_pace", C.c_int, [i64, C.c_int]),
     ("cf_rk4_tb_f32s_region", st, [i64, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("cf_rk4_tb_set_chunk", st, [i64, C.c_int]),
+    ("cf_rk4_tb_get_chunk", C.c_int, [i64]),
     ("cf_ensemble_create", st, [C.c_int, vp, vp, dbl, dbl, C.POINTER(Cosmology), C.c_int, C.POINTER(vp)]),
     ("cf_ensemble_destroy", st, [vp]),
     ("cf_ensemble_sites", i64, [vp]),

@@ -149,7 +149,7 @@ static TbGeom tb_geom(int64_t N, int A, int RB = TBR, int RC = TBR, int64_t a_lo
 // N=512 chunk 256 (the rule's pick) and 171 within 1-3% of each other, one chunk
 // 96 / 77.
 static int tb_default_chunk(int64_t N, int RB = TBR, int RC = TBR, int64_t planes = -1,
-                            int64_t *cost_out = nullptr) {
+                            int64_t *cost_out = nullptr, std::vector<int> *near = nullptr, int near_pct = 0) {
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
@@ -171,6 +171,9 @@ static int tb_default_chunk(int64_t N, int RB = TBR, int RC = TBR, int64_t plane
   for (int64_t k = 1; k <= kmax; ++k)
     if (cost[k] >= 0 && cost[k] * 100 <= best * 102) pick = k;
   if (cost_out) *cost_out = best;
+  if (near)  // every chunk length within near_pct percent of the best cost, longest first
+    for (int64_t k = 1; k <= kmax; ++k)
+      if (cost[k] >= 0 && cost[k] * 100 <= best * (100 + near_pct)) near->push_back((int)cdiv(planes, k));
   return (int)cdiv(planes, pick);
 }
 
@@ -206,11 +209,25 @@ static void tb_f32s_shape(int64_t N, int64_t planes, int *RB, int *RC) {
   *RB = square ? TB_F32C_RB : TB_F32S_RB;
   *RC = square ? TB_F32C_RC : TB_F32S_RC;
 }
+// Default chunk of the single-precision-stencil step per device and lattice size, set by
+// cf_rk4_tb_tune_pace (chunk <= 0) or cf_rk4_tb_set_chunk; absent: the cost model's.
+static std::mutex g_chunk_mu;
+static std::map<std::pair<int, int64_t>, int> g_chunk;
+
+static int tb_f32s_tuned_chunk(int64_t N) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> lock(g_chunk_mu);
+  const auto it = g_chunk.find(std::make_pair(dev, N));
+  return it == g_chunk.end() ? 0 : it->second;
+}
+
 // ... and of the single-precision-stencil kernel (sa) or the FP32-comp one.
 static TbGeom tb_comp_geom(bool sa, int64_t N, int chunk) {
   if (!sa) return tb_f32c_geom(N, chunk);
   int RB = 0, RC = 0;
   tb_f32s_shape(N, N, &RB, &RC);
+  if (chunk <= 0) chunk = tb_f32s_tuned_chunk(N);
   return tb_geom(N, chunk > 0 ? chunk : tb_default_chunk(N, RB, RC), RB, RC);
 }
 
@@ -559,26 +576,19 @@ cf_status cf_rk4_step_tb_f32s_slab(const cf_kg_params *p, int64_t n_planes, cons
 // larger pace); unpaced wins if it is as fast.
 constexpr int TB_PACE_MARGIN = 2;
 
-cf_status cf_rk4_tb_tune_pace(const cf_kg_params *p, const float *hi, const float *lo, double t, double dt, int chunk,
-                              void *work, cf_stream stream, int *pace_out) {
-  CF_TRY(tb_check(p));
-  const size_t n = 2 * (size_t)p->N * p->N * p->N;
-  const size_t wb = 2 * n * sizeof(float) + 2 * TB_ALIGN_SLACK;  // cf_rk4_tb_work_bytes
-  CF_REQUIRE(hi && lo && work && disjoint(hi, lo, n * sizeof(float)) &&
-                 disjoint(hi, n * sizeof(float), work, wb) && disjoint(lo, n * sizeof(float), work, wb),
-             "cf_rk4_tb_tune_pace: null or overlapping buffer");
-  hipStream_t st = S(stream);
-  TbGeom g = tb_comp_geom(true, p->N, chunk);
-  float *ho = same_residue(work, hi), *lw = same_residue(ho + n, lo);
-  const TbStep k = tb_step_host(p, t, dt);
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  CF_HIP(hipEventCreate(&e0));
-  if (hipEventCreate(&e1) != hipSuccess) {
-    (void)hipEventDestroy(e0);
-    CF_REQUIRE(false, "cf_rk4_tb_tune_pace: hipEventCreate failed");
-  }
+// Chunk lengths within this many percent of the cost model's best that the pace tuner
+// tries when the caller leaves the chunk to it. At N=512 (24 x 40 regions) these are 512,
+// 256, 171 and 128 planes; measured tuned: 109.0-110.4, 105.9-107.2, 108.4-109.3 and
+// 106.9-108.5 GSite*step/s: the drift of the blocks, which the cost model does not
+// see, depends on the chunk length and how the rounds of blocks line up.
+constexpr int TB_CHUNK_NEAR_PCT = 5;
+
+// Pace of launches over geometry g: the pick and its score (the slowest ms per step over
+// the picked pace and the TB_PACE_MARGIN paces below it; the unpaced ms for pick 0).
+static cf_status tb_tune_pace_geom(TbGeom g, const TbStep &k, const float *hi, const float *lo, float *ho,
+                                   float *lw, hipStream_t st, hipEvent_t e0, hipEvent_t e1, int *pick_out,
+                                   float *score_out) {
   std::map<int, float> ms;  // pace -> ms per step
-  cf_status rc = CF_OK;
   auto time_pace = [&](int pace) -> cf_status {
     if (ms.count(pace)) return CF_OK;
     g.pace = pace;
@@ -598,15 +608,12 @@ cf_status cf_rk4_tb_tune_pace(const cf_kg_params *p, const float *hi, const floa
   };
   // coarse grid 0.64 .. 3.2 us per plane, then every tick from below the coarse best to
   // the margin above it
-  for (int pace = 0; pace <= 320 && rc == CF_OK; pace += (pace == 0 ? 64 : 8)) rc = time_pace(pace);
+  for (int pace = 0; pace <= 320; pace += (pace == 0 ? 64 : 8)) CF_TRY(time_pace(pace));
   int coarse = 0;
   for (const auto &kv : ms)
     if (kv.second < ms[coarse]) coarse = kv.first;
   if (coarse > 0)
-    for (int pace = coarse - 8; pace <= coarse + 8 + TB_PACE_MARGIN && rc == CF_OK; ++pace) rc = time_pace(pace);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  CF_TRY(rc);
+    for (int pace = coarse - 8; pace <= coarse + 8 + TB_PACE_MARGIN; ++pace) CF_TRY(time_pace(pace));
   int pick = 0;
   float pick_worst = ms[0];
   for (const auto &kv : ms) {
@@ -623,8 +630,60 @@ cf_status cf_rk4_tb_tune_pace(const cf_kg_params *p, const float *hi, const floa
       pick_worst = worst;
     }
   }
-  CF_TRY(tb_pace_store(g, pick));
-  if (pace_out) *pace_out = pick;
+  *pick_out = pick;
+  *score_out = pick_worst;
+  return CF_OK;
+}
+
+cf_status cf_rk4_tb_tune_pace(const cf_kg_params *p, const float *hi, const float *lo, double t, double dt, int chunk,
+                              void *work, cf_stream stream, int *pace_out) {
+  CF_TRY(tb_check(p));
+  const size_t n = 2 * (size_t)p->N * p->N * p->N;
+  const size_t wb = 2 * n * sizeof(float) + 2 * TB_ALIGN_SLACK;  // cf_rk4_tb_work_bytes
+  CF_REQUIRE(hi && lo && work && disjoint(hi, lo, n * sizeof(float)) &&
+                 disjoint(hi, n * sizeof(float), work, wb) && disjoint(lo, n * sizeof(float), work, wb),
+             "cf_rk4_tb_tune_pace: null or overlapping buffer");
+  hipStream_t st = S(stream);
+  int RB = 0, RC = 0;
+  tb_f32s_shape(p->N, p->N, &RB, &RC);
+  std::vector<int> chunks;
+  if (chunk > 0)
+    chunks.push_back(chunk);
+  else
+    tb_default_chunk(p->N, RB, RC, p->N, nullptr, &chunks, TB_CHUNK_NEAR_PCT);
+  float *ho = same_residue(work, hi), *lw = same_residue(ho + n, lo);
+  const TbStep k = tb_step_host(p, t, dt);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  CF_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    CF_REQUIRE(false, "cf_rk4_tb_tune_pace: hipEventCreate failed");
+  }
+  cf_status rc = CF_OK;
+  int best_chunk = 0, best_pace = 0;
+  float best_score = 0.0f;
+  for (size_t i = 0; i < chunks.size() && rc == CF_OK; ++i) {
+    const TbGeom g = tb_geom(p->N, chunks[i], RB, RC);
+    int pick = 0;
+    float score = 0.0f;
+    rc = tb_tune_pace_geom(g, k, hi, lo, ho, lw, st, e0, e1, &pick, &score);
+    if (rc == CF_OK) rc = tb_pace_store(g, pick);
+    if (rc == CF_OK && (i == 0 || score < best_score)) {
+      best_chunk = g.A;
+      best_pace = pick;
+      best_score = score;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  CF_TRY(rc);
+  if (chunk <= 0) {
+    int dev = 0;
+    CF_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_chunk_mu);
+    g_chunk[std::make_pair(dev, p->N)] = best_chunk;
+  }
+  if (pace_out) *pace_out = best_pace;
   return CF_OK;
 }
 
@@ -632,6 +691,24 @@ cf_status cf_rk4_tb_set_pace(int64_t N, int chunk, int pace) {
   CF_REQUIRE(valid_N(N), "cf_rk4_tb_set_pace: unsupported lattice size N=%lld", (long long)N);
   CF_REQUIRE(pace >= 0 && pace <= 100000, "cf_rk4_tb_set_pace: pace %d outside 0..100000", pace);
   return tb_pace_store(tb_comp_geom(true, N, chunk), pace);
+}
+
+cf_status cf_rk4_tb_set_chunk(int64_t N, int chunk) {
+  CF_REQUIRE(valid_N(N), "cf_rk4_tb_set_chunk: unsupported lattice size N=%lld", (long long)N);
+  CF_REQUIRE(chunk >= 0 && chunk <= N, "cf_rk4_tb_set_chunk: chunk %d outside 0..N", chunk);
+  int dev = 0;
+  CF_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_chunk_mu);
+  if (chunk == 0)
+    g_chunk.erase(std::make_pair(dev, N));
+  else
+    g_chunk[std::make_pair(dev, N)] = chunk;
+  return CF_OK;
+}
+
+int cf_rk4_tb_get_chunk(int64_t N) {
+  if (!valid_N(N)) return -1;
+  return tb_comp_geom(true, N, 0).A;
 }
 
 cf_status cf_rk4_tb_f32s_region(int64_t N, int *rows, int *cols) {

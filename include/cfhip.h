@@ -173,12 +173,23 @@ cf_status cf_rk4_run_tb_f32s(const cf_kg_params *p, float *hi_dev, float *lo_dev
  * plane (state hi/lo at time t, unchanged; work as for cf_rk4_run_tb_f32s) and keeps
  * the pace P whose slowest of P-2..P is fastest (the rate falls just below the best
  * pace, so the pick keeps a margin above it; unpaced if that is as fast), and
- * returns it in *pace_out (may be null); it synchronises the stream. Pacing changes
- * timing only, never results. cf_rk4_tb_get_pace returns -1 for an unsupported N. */
+ * returns it in *pace_out (may be null); it synchronises the stream. With chunk <= 0
+ * it does so for each chunk length whose launch the cost model rates within 5% of its
+ * best (at N=512 four of them, about 1400 steps in all), keeps every chunk's pace, and
+ * makes the chunk with the fastest score the default for N on this device (the chunk
+ * that chunk <= 0 means from then on in the f32s runs, steps and pace calls), its pace
+ * the one returned. Pacing and chunking change timing only, never results.
+ * cf_rk4_tb_get_pace returns -1 for an unsupported N. */
 cf_status cf_rk4_tb_tune_pace(const cf_kg_params *p, const float *hi_dev, const float *lo_dev, double t, double dt,
                               int chunk, void *work_dev, cf_stream stream, int *pace_out);
 cf_status cf_rk4_tb_set_pace(int64_t N, int chunk, int pace);
 int cf_rk4_tb_get_pace(int64_t N, int chunk);
+/* Default chunk (planes per block along a) of the f32s step for N on the current device
+ * (no reference counterpart): cf_rk4_tb_set_chunk sets it (0: back to the launch cost
+ * model's), cf_rk4_tb_get_chunk returns the chunk that chunk <= 0 resolves to (tuned,
+ * set or the cost model's), -1 for an unsupported N. */
+cf_status cf_rk4_tb_set_chunk(int64_t N, int chunk);
+int cf_rk4_tb_get_chunk(int64_t N);
 /* Region shape (rows along b x columns along c, halo included) over which
  * cf_rk4_run_tb_f32s and cf_rk4_step_tb_f32s cover an N^3 lattice on the current device:
  * 24 x 40 or 32 x 32 (no reference counterpart; the kernel's name carries it). */

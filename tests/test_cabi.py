@@ -131,6 +131,10 @@ def test_bad_arguments_rejected_before_device_work(lib):
     rows, cols = C.c_int(0), C.c_int(0)
     assert L.cf_rk4_tb_f32s_region(15, C.byref(rows), C.byref(cols)) == lib.CF_E_INVALID
     assert L.cf_rk4_tb_f32s_region(16, None, C.byref(cols)) == lib.CF_E_INVALID
+    assert L.cf_rk4_tb_set_chunk(15, 4) == lib.CF_E_INVALID
+    assert L.cf_rk4_tb_set_chunk(16, 17) == lib.CF_E_INVALID
+    assert L.cf_rk4_tb_set_chunk(16, -1) == lib.CF_E_INVALID
+    assert L.cf_rk4_tb_get_chunk(15) == -1
     pace = C.c_int(7)
     assert L.cf_rk4_tb_tune_pace(C.byref(p), None, x0, 1.0, 0.1, 0, fake, None, C.byref(pace)) == lib.CF_E_INVALID
     assert pace.value == 7

@@ -92,12 +92,17 @@ def test_f32s_tune_pace_keeps_state(cfl, O):
     hi, lo = split(rand_state(N, 26, amp=0.5))
     hd, ld = dev(hi.copy()), dev(lo.copy())
     work = cfl.alloc_tb_work(N, cfl.CF_PREC_F32S)
+    model_chunk = cfl.rk4_tb_get_chunk(N)
     pace = cfl.rk4_tb_tune_pace(p, hd, ld, 1.0, 0.05, work)
-    print(f"tuned pace N={N}: {pace}", flush=True)
-    assert 0 <= pace <= 332 and cfl.rk4_tb_get_pace(N) == pace
+    chunk = cfl.rk4_tb_get_chunk(N)
+    print(f"tuned pace N={N}: {pace}, chunk {chunk} (cost model {model_chunk})", flush=True)
+    assert 0 <= pace <= 332 and cfl.rk4_tb_get_pace(N) == pace and cfl.rk4_tb_get_pace(N, chunk) == pace
+    assert 1 <= chunk <= N
     assert np.array_equal(host(hd), hi) and np.array_equal(host(ld), lo)
-    cfl.rk4_run_tb_f32s(p, hd, ld, 1.0, 0, 2, 0.05, work)  # with the tuned pace
+    cfl.rk4_run_tb_f32s(p, hd, ld, 1.0, 0, 2, 0.05, work)  # with the tuned chunk and pace
     cfl.rk4_tb_set_pace(N, 0)
+    cfl.rk4_tb_set_chunk(N, 0)
+    assert cfl.rk4_tb_get_chunk(N) == model_chunk
     eq = O.Equation(N, float(N), 1.0, 0.0, O.Cosmo.radiation(**FRW))
     for k in range(2):
         eq.rk4_step_f32s(hi, lo, 1.0 + k * 0.05, 0.05)
