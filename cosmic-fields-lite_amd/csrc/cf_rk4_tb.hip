@@ -660,19 +660,45 @@ cf_status cf_rk4_tb_tune_pace(const cf_kg_params *p, const float *hi, const floa
     CF_REQUIRE(false, "cf_rk4_tb_tune_pace: hipEventCreate failed");
   }
   cf_status rc = CF_OK;
-  int best_chunk = 0, best_pace = 0;
-  float best_score = 0.0f;
+  std::vector<TbGeom> picks;
   for (size_t i = 0; i < chunks.size() && rc == CF_OK; ++i) {
-    const TbGeom g = tb_geom(p->N, chunks[i], RB, RC);
-    int pick = 0;
+    TbGeom g = tb_geom(p->N, chunks[i], RB, RC);
     float score = 0.0f;
-    rc = tb_tune_pace_geom(g, k, hi, lo, ho, lw, st, e0, e1, &pick, &score);
-    if (rc == CF_OK) rc = tb_pace_store(g, pick);
-    if (rc == CF_OK && (i == 0 || score < best_score)) {
-      best_chunk = g.A;
-      best_pace = pick;
-      best_score = score;
+    rc = tb_tune_pace_geom(g, k, hi, lo, ho, lw, st, e0, e1, &g.pace, &score);
+    if (rc == CF_OK) rc = tb_pace_store(g, g.pace);
+    picks.push_back(g);
+  }
+  // Between chunks: each at its pick, timed again over longer windows (the fastest of
+  // three windows of five launches; a pace tuning's short windows and margin rank the
+  // paces of one chunk, not chunks whose rate curves differ in steepness).
+  auto time_pick = [&](const TbGeom &g, float *ms) -> cf_status {
+    *ms = 1e30f;
+    for (int w = 0; w < 3; ++w) {
+      CF_TRY(tb_launch_sa(g, k, hi, lo, ho, lw, st));  // a gap-free start for the window
+      CF_HIP(hipEventRecord(e0, st));
+      for (int r = 0; r < 5; ++r) CF_TRY(tb_launch_sa(g, k, hi, lo, ho, lw, st));
+      CF_HIP(hipEventRecord(e1, st));
+      CF_HIP(hipEventSynchronize(e1));
+      float el = 0.0f;
+      CF_HIP(hipEventElapsedTime(&el, e0, e1));
+      if (el / 5.0f < *ms) *ms = el / 5.0f;
     }
+    return CF_OK;
+  };
+  int best_chunk = 0, best_pace = 0;
+  float best_ms = 0.0f;
+  for (size_t i = 0; i < picks.size() && rc == CF_OK && chunks.size() > 1; ++i) {
+    float ms = 0.0f;
+    rc = time_pick(picks[i], &ms);
+    if (rc == CF_OK && (i == 0 || ms < best_ms)) {
+      best_chunk = picks[i].A;
+      best_pace = picks[i].pace;
+      best_ms = ms;
+    }
+  }
+  if (chunks.size() == 1 && rc == CF_OK) {
+    best_chunk = picks[0].A;
+    best_pace = picks[0].pace;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
