@@ -13,8 +13,8 @@ float stage inputs, the increment and the state update in double); run B is full
 FP64. They are timed with one temporally blocked kernel per step (tb_*) and, for f32c
 and FP64, with four fused stage kernels per step; every path produces the bits of
 its scheme (FP64: the reference's arithmetic order). Before the f32s path is timed,
-the library's tuner picks the pace of its lock-step for this lattice (untimed set-up,
-about 0.4 s at N=512). The headline `value` is run A's fastest path. Inputs are resident in HBM before the timed region, and the GPU
+the library's tuner picks the chunk length and the pace of its lock-step for this
+lattice (untimed set-up, about 2.7 s at N=512). The headline `value` is run A's fastest path. Inputs are resident in HBM before the timed region, and the GPU
 has run 0.3 s of untimed steps before the first path, so that every path is timed at
 settled clocks (each then does its own --warmup steps). The line
 also carries the roofline of the headline kernel (HIP events on the launch

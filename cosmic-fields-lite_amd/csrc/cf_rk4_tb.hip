@@ -57,9 +57,9 @@ __device__ __forceinline__ TbUnit tb_unit_of(const TbGeom &g, int bid) {
 // tiles, whose c edges fall on 128-byte lines; 960 threads) 83.0-83.4; 25 x 40 81.4-82.1.
 constexpr int TB_F32C_RB = 32, TB_F32C_RC = 32;
 // Region shape of the single-precision-stencil kernel: 24 x 40 (16 x 32 tiles, 960 of the
-// 1024 threads; the tiles' c edges fall on 128-byte lines, and N=512 makes 512 tiles,
-// 1024 blocks in two chunks, four full rounds over 256 CUs). Measured at N=512
-// (GSite*step/s, same bits): unpaced 101.7 against 76.7 for 32 x 32, paced 107.0 (at 115
+// 1024 threads; the tiles' c edges fall on 128-byte lines, and N=512 makes 512 tiles).
+// Measured at N=512 with 256-plane chunks (1024 blocks, four full rounds over 256 CUs;
+// GSite*step/s, same bits): unpaced 101.7 against 76.7 for 32 x 32, paced 107.0 (at 115
 // ticks per plane) against 97.3 (at 128).
 constexpr int TB_F32S_RB = 24, TB_F32S_RC = 40;
 
